@@ -1,0 +1,166 @@
+"""The per-stage functions of the drop-in API.
+
+Each function has the name and signature of its mujoco_warp counterpart
+(mujoco_warp/__init__.py:18-47), mutates `d` in place on torch's current
+stream and returns without synchronising.  All compute runs in
+libmjx_warp_amd.so; there is no Python or CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+from . import _native
+from .types import Data, Model
+
+
+def _ptr(t):
+  return ctypes.c_void_p(t.data_ptr())
+
+
+def kinematics(m: Model, d: Data):
+  """Forward kinematics (smooth.py:32-145)."""
+  _native.call("mjh_kinematics", m, d)
+
+
+def com_pos(m: Model, d: Data):
+  """Subtree COM, com-based inertias and motion dofs (smooth.py:148-245)."""
+  _native.call("mjh_com_pos", m, d)
+
+
+def crb(m: Model, d: Data):
+  """Composite rigid-body inertia and qM (smooth.py:248-314)."""
+  _native.call("mjh_crb", m, d)
+
+
+def factor_m(m: Model, d: Data):
+  """Cholesky factor of qM into qLD (smooth.py:392-395)."""
+  _native.call("mjh_factor_m", m, d)
+
+
+def solve_m(m: Model, d: Data, x, y):
+  """x = inv(M) y using qLD (smooth.py:693-696)."""
+  _native.call("mjh_solve_m", m, d, _ptr(x), _ptr(y))
+
+
+def rne(m: Model, d: Data):
+  """Bias forces by recursive Newton-Euler (smooth.py:398-466)."""
+  _native.call("mjh_rne", m, d)
+
+
+def transmission(m: Model, d: Data):
+  """Actuator lengths and moments (smooth.py:469-536)."""
+  _native.call("mjh_transmission", m, d)
+  return d
+
+
+def com_vel(m: Model, d: Data):
+  """cvel and cdof_dot (smooth.py:539-606)."""
+  _native.call("mjh_com_vel", m, d)
+
+
+def passive(m: Model, d: Data):
+  """Joint springs and dof dampers (passive.py:27-108)."""
+  _native.call("mjh_passive", m, d)
+
+
+def mul_m(m: Model, d: Data, res, vec, skip):
+  """res = M vec for worlds with skip == False (support.py:35-137)."""
+  _native.call("mjh_mul_m", m, d, _ptr(res), _ptr(vec), _ptr(skip) if skip is not None else None)
+
+
+def xfrc_accumulate(m: Model, d: Data, qfrc):
+  """qfrc += J' xfrc_applied (support.py:140-172)."""
+  _native.call("mjh_xfrc_accumulate", m, d, _ptr(qfrc))
+
+
+def collision(m: Model, d: Data):
+  """Broadphase, narrowphase and contact parameters (collision_driver.py:525-547)."""
+  _native.call("mjh_collision", m, d)
+
+
+def nxn_broadphase(m: Model, d: Data):
+  """All-pairs broadphase (collision_driver.py:458-512)."""
+  _native.call("mjh_nxn_broadphase", m, d)
+
+
+def sap_broadphase(m: Model, d: Data):
+  """Sweep-and-prune broadphase (collision_driver.py:350-455)."""
+  _native.call("mjh_sap_broadphase", m, d)
+  return d
+
+
+def primitive_narrowphase(m: Model, d: Data):
+  """Primitive-pair narrowphase (collision_primitive.py:271-274)."""
+  _native.call("mjh_primitive_narrowphase", m, d)
+
+
+def make_constraint(m: Model, d: Data):
+  """Joint-limit and pyramidal contact rows (constraint.py:213-240)."""
+  _native.call("mjh_make_constraint", m, d)
+
+
+def solve(m: Model, d: Data):
+  """CG / Newton constraint solver (solver.py:872-1000)."""
+  _native.call("mjh_solve", m, d)
+
+
+def fwd_position(m: Model, d: Data):
+  """Position-dependent computations (forward.py:408-420)."""
+  _native.call("mjh_fwd_position", m, d)
+
+
+def fwd_velocity(m: Model, d: Data):
+  """Velocity-dependent computations (forward.py:423-500)."""
+  _native.call("mjh_fwd_velocity", m, d)
+
+
+def fwd_actuation(m: Model, d: Data):
+  """Actuator forces (forward.py:503-641)."""
+  _native.call("mjh_fwd_actuation", m, d)
+
+
+def fwd_acceleration(m: Model, d: Data):
+  """qfrc_smooth and qacc_smooth (forward.py:644-661)."""
+  _native.call("mjh_fwd_acceleration", m, d)
+
+
+def forward(m: Model, d: Data):
+  """Forward dynamics (forward.py:664-679)."""
+  _native.call("mjh_forward", m, d)
+
+
+def euler(m: Model, d: Data):
+  """Semi-implicit Euler with implicit damping (forward.py:165-242)."""
+  _native.call("mjh_euler", m, d)
+  d.time = d.time + m.opt.timestep
+
+
+def implicit(m: Model, d: Data):
+  """implicitfast integrator (forward.py:245-405)."""
+  if m.opt.is_sparse:
+    raise AssertionError("implicit integration requires a dense mass matrix")
+  _native.call("mjh_implicit", m, d)
+  d.time = d.time + m.opt.timestep
+
+
+def step(m: Model, d: Data):
+  """Advance simulation by one step (forward.py:682-695), one launch per stage."""
+  if m.opt.integrator not in (0, 3):
+    raise NotImplementedError(f"integrator {m.opt.integrator} not implemented.")
+  _native.call("mjh_step", m, d)
+  d.time = d.time + m.opt.timestep
+
+
+def step_fused(m: Model, d: Data):
+  """The same step computed by a single fused launch per step."""
+  if m.opt.integrator not in (0, 3):
+    raise NotImplementedError(f"integrator {m.opt.integrator} not implemented.")
+  _native.call("mjh_step_fused", m, d)
+  d.time = d.time + m.opt.timestep
+
+
+def rollout(m: Model, d: Data, nstep: int, fused: bool = True):
+  """nstep consecutive steps issued by one library call."""
+  _native.call("mjh_rollout", m, d, ctypes.c_int(int(nstep)), ctypes.c_int(int(bool(fused))))
+  d.time = d.time + nstep * m.opt.timestep

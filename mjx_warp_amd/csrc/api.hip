@@ -1,0 +1,387 @@
+// Kernels and extern "C" entry points of libmjx_warp_amd.so.
+//
+// Every stage kernel is launched with 64-lane workgroups (one wavefront),
+// each holding 64/TEAM worlds; the grid covers nworld.  The plain step
+// (mjh_step) launches one kernel per reference stage; the fused step
+// (mjh_step_fused) runs every stage of a world inside one launch.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "collision.hpp"
+#include "constraint.hpp"
+#include "forward.hpp"
+#include "smooth.hpp"
+#include "solver.hpp"
+
+using namespace mjh;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+constexpr int kTeam = 64;
+
+// ---- stage functors ----
+struct StKinematics {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    kinematics(m, d, t);
+  }
+};
+struct StComPos {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    com_pos(m, d, t);
+  }
+};
+struct StCrb {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    crb(m, d, t);
+  }
+};
+struct StFactorM {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
+    factor_m(m, d, l, t);
+  }
+};
+struct StRne {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    rne(m, d, t);
+  }
+};
+struct StTransmission {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    transmission(m, d, t);
+  }
+};
+struct StComVel {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    com_vel(m, d, t);
+  }
+};
+struct StPassive {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    passive(m, d, t);
+  }
+};
+struct StNxn {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    nxn_broadphase(m, d, t);
+  }
+};
+struct StSap {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
+    sap_broadphase(m, d, l, t);
+  }
+};
+struct StBroadphase {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
+    if (m.ngeom <= 100) nxn_broadphase(m, d, t);
+    else sap_broadphase(m, d, l, t);
+  }
+};
+struct StNarrowphase {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    narrowphase(m, d, t);
+  }
+};
+struct StMakeConstraint {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    make_constraint(m, d, t);
+  }
+};
+struct StFwdVelocity {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    fwd_velocity(m, d, t);
+  }
+};
+struct StFwdActuation {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    fwd_actuation(m, d, t);
+  }
+};
+struct StFwdAcceleration {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
+    fwd_acceleration(m, d, l, t);
+  }
+};
+struct StSolve {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
+    solve(m, d, l, t);
+  }
+};
+struct StQaccSmooth {  // forward.py:676-677 (njmax == 0)
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
+    if (!t.valid) return;
+    for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] = d.qacc_smooth[t.world * m.nv + i];
+  }
+};
+struct StEuler {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
+    euler(m, d, l, t);
+  }
+};
+struct StImplicit {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
+    implicit(m, d, l, t);
+  }
+};
+
+// The whole step of one world (forward.py:682-695 / 664-679).
+struct StFusedStep {
+  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
+    kinematics(m, d, t);
+    com_pos(m, d, t);
+    crb(m, d, t);
+    factor_m(m, d, l, t);
+    sync();
+    collision(m, d, l, t);
+    make_constraint(m, d, t);
+    sync();
+    transmission(m, d, t);
+    fwd_velocity(m, d, t);
+    fwd_actuation(m, d, t);
+    fwd_acceleration(m, d, l, t);
+    if (d.njmax == 0) {
+      if (t.valid)
+        for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] = d.qacc_smooth[t.world * m.nv + i];
+      sync();
+    } else {
+      solve(m, d, l, t);
+    }
+    sync();
+    if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
+    else implicit(m, d, l, t);
+  }
+};
+
+template <int TEAM, class S>
+__global__ void __launch_bounds__(64) k_stage(mjhModel m, mjhData d, int lds_per_world) {
+  extern __shared__ float smem[];
+  Team<TEAM> t(d.nworld);
+  S()(m, d, smem + t.slot * lds_per_world, t);
+}
+
+// counter reset without memset nodes (capture-safe):
+// which bit 0: ncollision + ncon, bit 1: nefc
+__global__ void k_reset(mjhData d, int which) {
+  if (threadIdx.x == 0) {
+    if (which & 1) {
+      d.ncollision[0] = 0;
+      d.ncon[0] = 0;
+    }
+    if (which & 2) d.nefc[0] = 0;
+  }
+}
+
+// res = M vec / x = inv(M) y / qfrc += J' xfrc for caller-provided vectors
+__global__ void __launch_bounds__(64) k_mul_m(mjhModel m, mjhData d, float* res, const float* vec,
+                                              const uint8_t* skip) {
+  Team<kTeam> t(d.nworld);
+  if (!t.valid) return;
+  if (skip && skip[t.world]) return;
+  const size_t o = (size_t)t.world * m.nv;
+  mul_m(m, d.qM + o * m.nv, res + o, vec + o, t);
+}
+__global__ void __launch_bounds__(64) k_solve_m(mjhModel m, mjhData d, float* x, const float* y, int ldsw) {
+  extern __shared__ float smem[];
+  Team<kTeam> t(d.nworld);
+  if (!t.valid) return;
+  const size_t o = (size_t)t.world * m.nv;
+  solve_blocks(m, d.qLD + o * m.nv, x + o, y + o, smem + t.slot * ldsw, t);
+}
+__global__ void __launch_bounds__(64) k_xfrc(mjhModel m, mjhData d, float* qfrc) {
+  Team<kTeam> t(d.nworld);
+  xfrc_accumulate(m, d, qfrc, t);
+}
+
+int lds_linalg(const mjhModel& m) { return m.nv * m.nv + m.nv; }
+int lds_sap(const mjhModel& m) {
+  int np = 1;
+  while (np < m.ngeom) np <<= 1;
+  return 2 * np + 4 + 4 * m.ngeom;
+}
+int lds_step(const mjhModel& m) {
+  const int a = lds_linalg(m), b = m.ngeom > 100 ? lds_sap(m) : 0;
+  return a > b ? a : b;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(MJH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return MJH_OK;
+}
+
+template <class S>
+int launch(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, const char* name) {
+  if (d.nworld <= 0) return MJH_OK;
+  constexpr int wpb = 64 / kTeam;
+  const int grid = (d.nworld + wpb - 1) / wpb;
+  const size_t bytes = (size_t)ldsw * wpb * sizeof(float);
+  hipLaunchKernelGGL((k_stage<kTeam, S>), dim3(grid), dim3(64), bytes, s, m, d, ldsw);
+  return check_launch(name);
+}
+
+int reset(const mjhData& d, hipStream_t s, int which) {
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, d, which);
+  return check_launch("reset");
+}
+
+#define TRY(x)                 \
+  do {                         \
+    int _rc = (x);             \
+    if (_rc != MJH_OK) return _rc; \
+  } while (0)
+
+int do_collision(const mjhModel& m, const mjhData& d, hipStream_t s) {
+  TRY(launch<StBroadphase>(m, d, s, m.ngeom > 100 ? lds_sap(m) : 0, "broadphase"));
+  return launch<StNarrowphase>(m, d, s, 0, "narrowphase");
+}
+
+int do_fwd_position(const mjhModel& m, const mjhData& d, hipStream_t s) {
+  TRY(launch<StKinematics>(m, d, s, 0, "kinematics"));
+  TRY(launch<StComPos>(m, d, s, 0, "com_pos"));
+  TRY(launch<StCrb>(m, d, s, 0, "crb"));
+  TRY(launch<StFactorM>(m, d, s, m.nv * m.nv, "factor_m"));
+  TRY(reset(d, s, 1));
+  TRY(do_collision(m, d, s));
+  if (!(m.opt.disableflags & mjhDSBL_CONSTRAINT)) {
+    TRY(reset(d, s, 2));
+    TRY(launch<StMakeConstraint>(m, d, s, 0, "make_constraint"));
+  }
+  return launch<StTransmission>(m, d, s, 0, "transmission");
+}
+
+int do_forward(const mjhModel& m, const mjhData& d, hipStream_t s) {
+  TRY(do_fwd_position(m, d, s));
+  TRY(launch<StFwdVelocity>(m, d, s, 0, "fwd_velocity"));
+  TRY(launch<StFwdActuation>(m, d, s, 0, "fwd_actuation"));
+  TRY(launch<StFwdAcceleration>(m, d, s, lds_linalg(m), "fwd_acceleration"));
+  if (d.njmax == 0) return launch<StQaccSmooth>(m, d, s, 0, "qacc_smooth");
+  return launch<StSolve>(m, d, s, lds_linalg(m), "solve");
+}
+
+int do_integrate(const mjhModel& m, const mjhData& d, hipStream_t s) {
+  if (m.opt.integrator == mjhINT_EULER) return launch<StEuler>(m, d, s, lds_linalg(m), "euler");
+  if (m.opt.integrator == mjhINT_IMPLICITFAST) return launch<StImplicit>(m, d, s, lds_linalg(m), "implicit");
+  return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
+}
+
+int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
+  if (m.opt.integrator != mjhINT_EULER && m.opt.integrator != mjhINT_IMPLICITFAST)
+    return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
+  TRY(reset(d, s, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3));
+  return launch<StFusedStep>(m, d, s, lds_step(m), "step_fused");
+}
+
+}  // namespace
+
+extern "C" {
+
+int mjh_abi_version(void) { return MJH_ABI_VERSION; }
+
+const char* mjh_last_error(void) { return g_last_error.c_str(); }
+
+int mjh_check(const mjhModel* m, const mjhData* d) {
+  if (!m || !d) return fail(MJH_ERR_INVALID, "null model or data");
+  if (m->opt.is_sparse) return fail(MJH_ERR_UNSUPPORTED, "sparse mass matrix is not implemented");
+  if (m->opt.cone != mjhCONE_PYRAMIDAL) return fail(MJH_ERR_UNSUPPORTED, "only the pyramidal cone is implemented");
+  if (m->opt.solver != mjhSOL_CG && m->opt.solver != mjhSOL_NEWTON)
+    return fail(MJH_ERR_UNSUPPORTED, "Solver: " + std::to_string(m->opt.solver) + " is unsupported.");
+  if (m->nlevel < 1 || m->nbody < 1) return fail(MJH_ERR_INVALID, "model has no bodies");
+  if (d->nworld < 0 || d->nconmax < 0 || d->njmax < 0) return fail(MJH_ERR_INVALID, "negative sizes");
+  if (d->qM_rows != m->nv || d->qM_cols != m->nv) return fail(MJH_ERR_INVALID, "qM must be dense [nworld, nv, nv]");
+  const size_t bytes = (size_t)lds_step(*m) * sizeof(float);
+  if (bytes > 65536) return fail(MJH_ERR_CAPACITY, "model too large for the dense LDS kernels (nv too large)");
+  return MJH_OK;
+}
+
+#define STAGE_ENTRY(fname, S, ldsexpr)                                            \
+  int fname(const mjhModel* m, const mjhData* d, mjhStream stream) {              \
+    return launch<S>(*m, *d, (hipStream_t)stream, (ldsexpr), #fname);             \
+  }
+
+STAGE_ENTRY(mjh_kinematics, StKinematics, 0)
+STAGE_ENTRY(mjh_com_pos, StComPos, 0)
+STAGE_ENTRY(mjh_crb, StCrb, 0)
+STAGE_ENTRY(mjh_factor_m, StFactorM, m->nv * m->nv)
+STAGE_ENTRY(mjh_rne, StRne, 0)
+STAGE_ENTRY(mjh_transmission, StTransmission, 0)
+STAGE_ENTRY(mjh_com_vel, StComVel, 0)
+STAGE_ENTRY(mjh_passive, StPassive, 0)
+STAGE_ENTRY(mjh_fwd_velocity, StFwdVelocity, 0)
+STAGE_ENTRY(mjh_fwd_actuation, StFwdActuation, 0)
+STAGE_ENTRY(mjh_fwd_acceleration, StFwdAcceleration, lds_linalg(*m))
+STAGE_ENTRY(mjh_solve, StSolve, lds_linalg(*m))
+
+int mjh_nxn_broadphase(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  TRY(reset(*d, (hipStream_t)stream, 1));
+  return launch<StNxn>(*m, *d, (hipStream_t)stream, 0, "nxn_broadphase");
+}
+int mjh_sap_broadphase(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  TRY(reset(*d, (hipStream_t)stream, 1));
+  return launch<StSap>(*m, *d, (hipStream_t)stream, lds_sap(*m), "sap_broadphase");
+}
+int mjh_primitive_narrowphase(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  return launch<StNarrowphase>(*m, *d, (hipStream_t)stream, 0, "primitive_narrowphase");
+}
+int mjh_collision(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  TRY(reset(*d, (hipStream_t)stream, 1));
+  return do_collision(*m, *d, (hipStream_t)stream);
+}
+int mjh_make_constraint(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  if (m->opt.disableflags & mjhDSBL_CONSTRAINT) return MJH_OK;
+  TRY(reset(*d, (hipStream_t)stream, 2));
+  return launch<StMakeConstraint>(*m, *d, (hipStream_t)stream, 0, "make_constraint");
+}
+int mjh_fwd_position(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  return do_fwd_position(*m, *d, (hipStream_t)stream);
+}
+int mjh_euler(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  return launch<StEuler>(*m, *d, (hipStream_t)stream, lds_linalg(*m), "euler");
+}
+int mjh_implicit(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  return launch<StImplicit>(*m, *d, (hipStream_t)stream, lds_linalg(*m), "implicit");
+}
+int mjh_forward(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  return do_forward(*m, *d, (hipStream_t)stream);
+}
+int mjh_step(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  if (m->opt.integrator != mjhINT_EULER && m->opt.integrator != mjhINT_IMPLICITFAST)
+    return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m->opt.integrator) + " not implemented.");
+  TRY(do_forward(*m, *d, (hipStream_t)stream));
+  return do_integrate(*m, *d, (hipStream_t)stream);
+}
+int mjh_step_fused(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  return do_step_fused(*m, *d, (hipStream_t)stream);
+}
+int mjh_rollout(const mjhModel* m, const mjhData* d, int nstep, int fused, mjhStream stream) {
+  for (int i = 0; i < nstep; i++) TRY(fused ? mjh_step_fused(m, d, stream) : mjh_step(m, d, stream));
+  return MJH_OK;
+}
+int mjh_solve_m(const mjhModel* m, const mjhData* d, float* x, const float* y, mjhStream stream) {
+  if (d->nworld <= 0) return MJH_OK;
+  const int ldsw = lds_linalg(*m);
+  hipLaunchKernelGGL(k_solve_m, dim3(d->nworld), dim3(64), ldsw * sizeof(float), (hipStream_t)stream, *m, *d, x, y,
+                     ldsw);
+  return check_launch("solve_m");
+}
+int mjh_mul_m(const mjhModel* m, const mjhData* d, float* res, const float* vec, const uint8_t* skip,
+              mjhStream stream) {
+  if (d->nworld <= 0) return MJH_OK;
+  hipLaunchKernelGGL(k_mul_m, dim3(d->nworld), dim3(64), 0, (hipStream_t)stream, *m, *d, res, vec, skip);
+  return check_launch("mul_m");
+}
+int mjh_xfrc_accumulate(const mjhModel* m, const mjhData* d, float* qfrc, mjhStream stream) {
+  if (d->nworld <= 0) return MJH_OK;
+  hipLaunchKernelGGL(k_xfrc, dim3(d->nworld), dim3(64), 0, (hipStream_t)stream, *m, *d, qfrc);
+  return check_launch("xfrc_accumulate");
+}
+
+}  // extern "C"

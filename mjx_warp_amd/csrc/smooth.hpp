@@ -1,0 +1,560 @@
+// Smooth dynamics of one world, executed by a team of lanes: forward
+// kinematics, subtree COM / com-frame inertias / motion axes, composite
+// rigid-body inertia and the dense joint-space mass matrix, its Cholesky
+// factor and solves, RNE bias forces, body velocities, passive forces and
+// actuator transmission.  Each function reproduces the arithmetic of the
+// reference stage cited in its comment; the reference's per-level launches
+// become per-level loops separated by sync(), and its atomic accumulations
+// become per-lane sequential sums (no two lanes write the same word).
+#pragma once
+
+#include "common.hpp"
+
+namespace mjh {
+
+MJH_DEV int level_end(const mjhModel& m, int l) {
+  return (l == m.nlevel - 1) ? m.nbody : m.body_treeadr[l + 1];
+}
+
+// ---------------------------------------------------------------------------
+// smooth.py:32-145 kinematics
+template <int TEAM>
+MJH_DEV void kinematics(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+  const int w = t.world;
+  const int nbody = m.nbody;
+  float* xpos = WPTR(d.xpos, w, nbody * 3);
+  float* xquat = WPTR(d.xquat, w, nbody * 4);
+  float* xmat = WPTR(d.xmat, w, nbody * 9);
+  float* xipos = WPTR(d.xipos, w, nbody * 3);
+  float* ximat = WPTR(d.ximat, w, nbody * 9);
+  float* xanchor = WPTR(d.xanchor, w, m.njnt * 3);
+  float* xaxis = WPTR(d.xaxis, w, m.njnt * 3);
+  const float* qpos = WPTR(d.qpos, w, m.nq);
+
+  if (t.valid && t.lane == 0) {
+    st3(xpos, V3{0.f, 0.f, 0.f});
+    st4(xquat, Q4{1.f, 0.f, 0.f, 0.f});
+    st3(xipos, V3{0.f, 0.f, 0.f});
+    M33 I = {{1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f}};
+    st9(xmat, I);
+    st9(ximat, I);
+  }
+  sync();
+  for (int l = 1; l < m.nlevel; l++) {
+    const int beg = m.body_treeadr[l], end = level_end(m, l);
+    for (int k = beg + t.lane; k < end && t.valid; k += TEAM) {
+      const int b = m.body_tree[k];
+      int jntadr = m.body_jntadr[b];
+      const int jntnum = m.body_jntnum[b];
+      V3 pos;
+      Q4 quat;
+      if (jntnum == 1 && m.jnt_type[jntadr] == mjhJNT_FREE) {
+        const int qadr = m.jnt_qposadr[jntadr];
+        pos = ld3(qpos + qadr);
+        quat = ld4(qpos + qadr + 3);
+        st3(xanchor + 3 * jntadr, pos);
+        st3(xaxis + 3 * jntadr, ld3(m.jnt_axis + 3 * jntadr));
+      } else {
+        const int pid = m.body_parentid[b];
+        pos = mulmv(ld9(xmat + 9 * pid), ld3(m.body_pos + 3 * b)) + ld3(xpos + 3 * pid);
+        quat = mul_quat(ld4(xquat + 4 * pid), ld4(m.body_quat + 4 * b));
+        for (int j = 0; j < jntnum; j++, jntadr++) {
+          const int qadr = m.jnt_qposadr[jntadr];
+          const int jt = m.jnt_type[jntadr];
+          const V3 jaxis = ld3(m.jnt_axis + 3 * jntadr);
+          const V3 jpos = ld3(m.jnt_pos + 3 * jntadr);
+          const V3 anchor = rot_vec_quat(jpos, quat) + pos;
+          const V3 axis = rot_vec_quat(jaxis, quat);
+          if (jt == mjhJNT_BALL) {
+            quat = mul_quat(quat, ld4(qpos + qadr));
+            pos = anchor - rot_vec_quat(jpos, quat);
+          } else if (jt == mjhJNT_SLIDE) {
+            pos = pos + axis * (qpos[qadr] - m.qpos0[qadr]);
+          } else if (jt == mjhJNT_HINGE) {
+            quat = mul_quat(quat, axis_angle_to_quat(jaxis, qpos[qadr] - m.qpos0[qadr]));
+            pos = anchor - rot_vec_quat(jpos, quat);
+          }
+          st3(xanchor + 3 * jntadr, anchor);
+          st3(xaxis + 3 * jntadr, axis);
+        }
+      }
+      st3(xpos + 3 * b, pos);
+      quat = qnormalize(quat);
+      st4(xquat + 4 * b, quat);
+      st9(xmat + 9 * b, quat_to_mat(quat));
+      st3(xipos + 3 * b, pos + rot_vec_quat(ld3(m.body_ipos + 3 * b), quat));
+      st9(ximat + 9 * b, quat_to_mat(mul_quat(quat, ld4(m.body_iquat + 4 * b))));
+    }
+    sync();
+  }
+  if (t.valid) {
+    float* gxpos = WPTR(d.geom_xpos, w, m.ngeom * 3);
+    float* gxmat = WPTR(d.geom_xmat, w, m.ngeom * 9);
+    for (int g = t.lane; g < m.ngeom; g += TEAM) {
+      const int b = m.geom_bodyid[g];
+      const Q4 q = ld4(xquat + 4 * b);
+      st3(gxpos + 3 * g, ld3(xpos + 3 * b) + rot_vec_quat(ld3(m.geom_pos + 3 * g), q));
+      st9(gxmat + 9 * g, quat_to_mat(mul_quat(q, ld4(m.geom_quat + 4 * g))));
+    }
+    float* sxpos = WPTR(d.site_xpos, w, m.nsite * 3);
+    float* sxmat = WPTR(d.site_xmat, w, m.nsite * 9);
+    for (int s = t.lane; s < m.nsite; s += TEAM) {
+      const int b = m.site_bodyid[s];
+      const Q4 q = ld4(xquat + 4 * b);
+      st3(sxpos + 3 * s, ld3(xpos + 3 * b) + rot_vec_quat(ld3(m.site_pos + 3 * s), q));
+      st9(sxmat + 9 * s, quat_to_mat(mul_quat(q, ld4(m.site_quat + 4 * s))));
+    }
+  }
+  sync();
+}
+
+// ---------------------------------------------------------------------------
+// smooth.py:148-245 com_pos
+template <int TEAM>
+MJH_DEV void com_pos(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+  const int w = t.world;
+  const int nbody = m.nbody;
+  float* sc = WPTR(d.subtree_com, w, nbody * 3);
+  const float* xipos = WPTR(d.xipos, w, nbody * 3);
+  if (t.valid)
+    for (int b = t.lane; b < nbody; b += TEAM) st3(sc + 3 * b, ld3(xipos + 3 * b) * m.body_mass[b]);
+  sync();
+  // accumulate subtrees into parents; the world body adds itself once more,
+  // which the doubled subtree_mass[0] cancels (io.py:249-252, smooth.py:238-241)
+  if (t.valid && t.lane < 3) {
+    const int k = t.lane;
+    for (int b = nbody - 1; b >= 0; b--) sc[3 * m.body_parentid[b] + k] += sc[3 * b + k];
+  }
+  sync();
+  if (t.valid)
+    for (int b = t.lane; b < nbody; b += TEAM) st3(sc + 3 * b, ld3(sc + 3 * b) * (1.0f / m.subtree_mass[b]));
+  sync();
+  if (t.valid) {
+    const float* ximat = WPTR(d.ximat, w, nbody * 9);
+    float* cinert = WPTR(d.cinert, w, nbody * 10);
+    for (int b = t.lane; b < nbody; b += TEAM) {
+      const M33 mat = ld9(ximat + 9 * b);
+      const V3 inert = ld3(m.body_inertia + 3 * b);
+      const float mass = m.body_mass[b];
+      const V3 dif = ld3(xipos + 3 * b) - ld3(sc + 3 * m.body_rootid[b]);
+      // mat * diag(inert) * mat'
+      float tmp[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          tmp[i][j] = mat.m[3 * i] * inert.x * mat.m[3 * j] + mat.m[3 * i + 1] * inert.y * mat.m[3 * j + 1] +
+                      mat.m[3 * i + 2] * inert.z * mat.m[3 * j + 2];
+      float* r = cinert + 10 * b;
+      r[0] = tmp[0][0] + mass * (dif.y * dif.y + dif.z * dif.z);
+      r[1] = tmp[1][1] + mass * (dif.x * dif.x + dif.z * dif.z);
+      r[2] = tmp[2][2] + mass * (dif.x * dif.x + dif.y * dif.y);
+      r[3] = tmp[0][1] - mass * dif.x * dif.y;
+      r[4] = tmp[0][2] - mass * dif.x * dif.z;
+      r[5] = tmp[1][2] - mass * dif.y * dif.z;
+      r[6] = mass * dif.x;
+      r[7] = mass * dif.y;
+      r[8] = mass * dif.z;
+      r[9] = mass;
+    }
+    const float* xaxis = WPTR(d.xaxis, w, m.njnt * 3);
+    const float* xanchor = WPTR(d.xanchor, w, m.njnt * 3);
+    const float* xmat = WPTR(d.xmat, w, nbody * 9);
+    float* cdof = WPTR(d.cdof, w, m.nv * 6);
+    for (int j = t.lane; j < m.njnt; j += TEAM) {
+      const int b = m.jnt_bodyid[j];
+      const int da = m.jnt_dofadr[j];
+      const int jt = m.jnt_type[j];
+      const V3 axis = ld3(xaxis + 3 * j);
+      const M33 xm = ld9(xmat + 9 * b);
+      const V3 offset = ld3(sc + 3 * m.body_rootid[b]) - ld3(xanchor + 3 * j);
+      if (jt == mjhJNT_FREE || jt == mjhJNT_BALL) {
+        int r0 = da;
+        if (jt == mjhJNT_FREE) {
+          for (int i = 0; i < 3; i++) {
+            float* c = cdof + 6 * (da + i);
+            for (int k = 0; k < 6; k++) c[k] = 0.0f;
+            c[3 + i] = 1.0f;
+          }
+          r0 = da + 3;
+        }
+        for (int i = 0; i < 3; i++) {
+          const V3 ax = mcol(xm, i);  // row i of xmat' = column i of xmat
+          const V3 lin = cross(ax, offset);
+          float* c = cdof + 6 * (r0 + i);
+          st3(c, ax);
+          st3(c + 3, lin);
+        }
+      } else if (jt == mjhJNT_SLIDE) {
+        float* c = cdof + 6 * da;
+        st3(c, V3{0.f, 0.f, 0.f});
+        st3(c + 3, axis);
+      } else {
+        float* c = cdof + 6 * da;
+        st3(c, axis);
+        st3(c + 3, cross(axis, offset));
+      }
+    }
+  }
+  sync();
+}
+
+// ---------------------------------------------------------------------------
+// smooth.py:248-314 crb (dense qM; the sparse layout is not built)
+template <int TEAM>
+MJH_DEV void crb(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+  const int w = t.world;
+  const int nbody = m.nbody, nv = m.nv;
+  float* crbw = WPTR(d.crb, w, nbody * 10);
+  const float* cinert = WPTR(d.cinert, w, nbody * 10);
+  if (t.valid)
+    for (int i = t.lane; i < nbody * 10; i += TEAM) crbw[i] = cinert[i];
+  sync();
+  // composite inertia: children into parents, bodies attached to the world
+  // are not added to the world body (smooth.py:255-261)
+  if (t.valid && t.lane < 10) {
+    const int k = t.lane;
+    for (int b = nbody - 1; b >= 1; b--) {
+      const int p = m.body_parentid[b];
+      if (p != 0) crbw[10 * p + k] += crbw[10 * b + k];
+    }
+  }
+  float* qM = WPTR(d.qM, w, nv * nv);
+  if (t.valid)
+    for (int i = t.lane; i < nv * nv; i += TEAM) qM[i] = 0.0f;
+  sync();
+  if (t.valid) {
+    const float* cdof = WPTR(d.cdof, w, nv * 6);
+    for (int i = t.lane; i < nv; i += TEAM) {
+      const S6 ci = ld6(cdof + 6 * i);
+      const S6 buf = inert_vec(crbw + 10 * m.dof_bodyid[i], ci);
+      qM[i * nv + i] = m.dof_armature[i] + dot6(ci, buf);
+      for (int j = m.dof_parentid[i]; j >= 0; j = m.dof_parentid[j]) {
+        const float v = dot6(ld6(cdof + 6 * j), buf);
+        qM[i * nv + j] = v;
+        qM[j * nv + i] = v;
+      }
+    }
+  }
+  sync();
+}
+
+// ---------------------------------------------------------------------------
+// Dense Cholesky of an n x n block (row stride ld) held in LDS: A = L L',
+// L overwrites the lower triangle and the strict upper triangle is zeroed,
+// matching wp.tile_cholesky as used by smooth.py:361-368.
+template <int TEAM>
+MJH_DEV void chol_lds(float* A, int n, int ld, const mjhModel& m, const Team<TEAM>& t) {
+  for (int k = 0; k < n; k++) {
+    const float dkk = sqrtf(A[k * ld + k]);
+    const float inv = 1.0f / dkk;
+    sync();
+    for (int i = k + 1 + t.lane; i < n; i += TEAM) A[i * ld + k] *= inv;
+    if (t.lane == 0) A[k * ld + k] = dkk;
+    sync();
+    // trailing update of the lower triangle of rows/cols k+1..n-1; the
+    // lower-triangle index tables enumerate (r, c) with c <= r
+    const int mrem = n - k - 1;
+    const int npair = mrem * (mrem + 1) / 2;
+    for (int p = t.lane; p < npair; p += TEAM) {
+      const int r = m.dof_tri_row[p] + k + 1;
+      const int c = m.dof_tri_col[p] + k + 1;
+      A[r * ld + c] -= A[r * ld + k] * A[c * ld + k];
+    }
+    sync();
+  }
+  for (int p = t.lane; p < n * n; p += TEAM) {
+    const int r = p / n, c = p % n;
+    if (c > r) A[r * ld + c] = 0.0f;
+  }
+  sync();
+}
+
+// x = inv(L L') x for an n-vector x in LDS, L lower (row stride ld) in LDS.
+template <int TEAM>
+MJH_DEV void cho_solve_lds(const float* L, int n, int ld, float* x, const Team<TEAM>& t) {
+  for (int k = 0; k < n; k++) {
+    const float xk = x[k] / L[k * ld + k];
+    sync();
+    for (int i = k + 1 + t.lane; i < n; i += TEAM) x[i] -= L[i * ld + k] * xk;
+    if (t.lane == 0) x[k] = xk;
+    sync();
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    const float xk = x[k] / L[k * ld + k];
+    sync();
+    for (int i = t.lane; i < k; i += TEAM) x[i] -= L[k * ld + i] * xk;
+    if (t.lane == 0) x[k] = xk;
+    sync();
+  }
+}
+
+// Iterates the dense per-tree blocks (qLD_tile / qLD_tileadr / qLD_tilesize,
+// io.py:150-162) and calls f(dofid, size) for each.
+template <class F>
+MJH_DEV void for_each_tree_block(const mjhModel& m, F f) {
+  for (int g = 0; g < m.nqLD_group; g++) {
+    const int beg = m.qLD_tileadr[g];
+    const int end = (g == m.nqLD_group - 1) ? m.nqLD_tile : m.qLD_tileadr[g + 1];
+    const int size = m.qLD_tilesize[g];
+    for (int k = beg; k < end; k++) f(m.qLD_tile[k], size);
+  }
+}
+
+// Factor the tree blocks of M (global, [nv, nv]) into L (global, may alias
+// a different array) through an LDS staging buffer of nv*nv floats.
+template <int TEAM>
+MJH_DEV void factor_blocks(const mjhModel& m, const float* M, float* L, float* lds, const Team<TEAM>& t) {
+  const int nv = m.nv;
+  for_each_tree_block(m, [&](int dof, int n) {
+    for (int p = t.lane; p < n * n; p += TEAM) lds[p] = M[(dof + p / n) * nv + dof + p % n];
+    sync();
+    chol_lds(lds, n, n, m, t);
+    for (int p = t.lane; p < n * n; p += TEAM) L[(dof + p / n) * nv + dof + p % n] = lds[p];
+    sync();
+  });
+}
+
+// x = inv(M) y with M = L L' given by its tree blocks (smooth.py:654-681).
+// x may alias y.  lds needs nv*nv + nv floats.
+template <int TEAM>
+MJH_DEV void solve_blocks(const mjhModel& m, const float* L, float* x, const float* y, float* lds,
+                          const Team<TEAM>& t) {
+  const int nv = m.nv;
+  for_each_tree_block(m, [&](int dof, int n) {
+    float* Ls = lds;
+    float* xs = lds + n * n;
+    for (int p = t.lane; p < n * n; p += TEAM) Ls[p] = L[(dof + p / n) * nv + dof + p % n];
+    for (int p = t.lane; p < n; p += TEAM) xs[p] = y[dof + p];
+    sync();
+    cho_solve_lds(Ls, n, n, xs, t);
+    for (int p = t.lane; p < n; p += TEAM) x[dof + p] = xs[p];
+    sync();
+  });
+}
+
+// smooth.py:392-395 factor_m (dense)
+template <int TEAM>
+MJH_DEV void factor_m(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
+  const int nv = m.nv;
+  if (!t.valid) return;  // padding slot; a workgroup is one wavefront
+  factor_blocks(m, WPTR(d.qM, t.world, nv * nv), WPTR(d.qLD, t.world, nv * nv), lds, t);
+}
+
+// ---------------------------------------------------------------------------
+// smooth.py:398-466 rne
+template <int TEAM>
+MJH_DEV void rne(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+  const int w = t.world;
+  const int nbody = m.nbody, nv = m.nv;
+  float* cacc = WPTR(d.rne_cacc, w, nbody * 6);
+  float* cfrc = WPTR(d.rne_cfrc, w, nbody * 6);
+  const float* cdof_dot = WPTR(d.cdof_dot, w, nv * 6);
+  const float* qvel = WPTR(d.qvel, w, nv);
+  if (t.valid && t.lane < 6) {
+    const bool nograv = m.opt.disableflags & mjhDSBL_GRAVITY;
+    cacc[t.lane] = (t.lane < 3 || nograv) ? 0.0f : -m.opt.gravity[t.lane - 3];
+  }
+  sync();
+  for (int l = 0; l < m.nlevel; l++) {
+    const int beg = m.body_treeadr[l], end = level_end(m, l);
+    for (int k = beg + t.lane; k < end && t.valid; k += TEAM) {
+      const int b = m.body_tree[k];
+      const int pid = m.body_parentid[b];
+      const int da = m.body_dofadr[b];
+      S6 a = ld6(cacc + 6 * pid);
+      for (int i = 0; i < m.body_dofnum[b]; i++) {
+        const float qv = qvel[da + i];
+#pragma unroll
+        for (int c = 0; c < 6; c++) a.v[c] += cdof_dot[6 * (da + i) + c] * qv;
+      }
+      st6(cacc + 6 * b, a);
+    }
+    sync();
+  }
+  if (t.valid) {
+    const float* cinert = WPTR(d.cinert, w, nbody * 10);
+    const float* cvel = WPTR(d.cvel, w, nbody * 6);
+    for (int b = t.lane; b < nbody; b += TEAM) {
+      const float* ci = cinert + 10 * b;
+      S6 f = inert_vec(ci, ld6(cacc + 6 * b));
+      const S6 v = ld6(cvel + 6 * b);
+      const S6 g = motion_cross_force(v, inert_vec(ci, v));
+#pragma unroll
+      for (int c = 0; c < 6; c++) f.v[c] += g.v[c];
+      st6(cfrc + 6 * b, f);
+    }
+  }
+  sync();
+  if (t.valid && t.lane < 6) {
+    const int k = t.lane;
+    for (int b = nbody - 1; b >= 0; b--) cfrc[6 * m.body_parentid[b] + k] += cfrc[6 * b + k];
+  }
+  sync();
+  if (t.valid) {
+    const float* cdof = WPTR(d.cdof, w, nv * 6);
+    float* bias = WPTR(d.qfrc_bias, w, nv);
+    for (int i = t.lane; i < nv; i += TEAM) bias[i] = dot6(ld6(cdof + 6 * i), ld6(cfrc + 6 * m.dof_bodyid[i]));
+  }
+  sync();
+}
+
+// ---------------------------------------------------------------------------
+// smooth.py:469-536 transmission
+template <int TEAM>
+MJH_DEV void transmission(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+  if (m.nu == 0) return;
+  const int w = t.world;
+  if (t.valid) {
+    const float* qpos = WPTR(d.qpos, w, m.nq);
+    float* length = WPTR(d.actuator_length, w, m.nu);
+    float* moment = WPTR(d.actuator_moment, w, m.nu * m.nv);
+    for (int a = t.lane; a < m.nu; a += TEAM) {
+      const int j = m.actuator_trnid[2 * a];
+      const int jt = m.jnt_type[j];
+      const int qadr = m.jnt_qposadr[j];
+      const int vadr = m.jnt_dofadr[j];
+      const int trn = m.actuator_trntype[a];
+      const float* gear = m.actuator_gear + 6 * a;
+      float* mrow = moment + a * m.nv;
+      if (trn != mjhTRN_JOINT && trn != mjhTRN_JOINTINPARENT) continue;
+      if (jt == mjhJNT_FREE) {
+        length[a] = 0.0f;
+        if (trn == mjhTRN_JOINTINPARENT) {
+          const Q4 qneg = quat_inv(ld4(qpos + qadr + 3));
+          const V3 ga = rot_vec_quat(V3{gear[3], gear[4], gear[5]}, qneg);
+          mrow[vadr + 0] = gear[0];
+          mrow[vadr + 1] = gear[1];
+          mrow[vadr + 2] = gear[2];
+          mrow[vadr + 3] = ga.x;
+          mrow[vadr + 4] = ga.y;
+          mrow[vadr + 5] = ga.z;
+        } else {
+          for (int i = 0; i < 6; i++) mrow[vadr + i] = gear[i];
+        }
+      } else if (jt == mjhJNT_BALL) {
+        const Q4 q = ld4(qpos + qadr);
+        const V3 aa = quat_to_vel(q);
+        V3 ga = V3{gear[0], gear[1], gear[2]};
+        if (trn == mjhTRN_JOINTINPARENT) ga = rot_vec_quat(ga, quat_inv(q));
+        length[a] = dot(aa, ga);
+        mrow[vadr + 0] = ga.x;
+        mrow[vadr + 1] = ga.y;
+        mrow[vadr + 2] = ga.z;
+      } else {
+        length[a] = qpos[qadr] * gear[0];
+        mrow[vadr] = gear[0];
+      }
+    }
+  }
+  sync();
+}
+
+// ---------------------------------------------------------------------------
+// smooth.py:539-606 com_vel
+template <int TEAM>
+MJH_DEV void com_vel(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+  const int w = t.world;
+  const int nbody = m.nbody, nv = m.nv;
+  float* cvel = WPTR(d.cvel, w, nbody * 6);
+  float* cdof_dot = WPTR(d.cdof_dot, w, nv * 6);
+  const float* cdof = WPTR(d.cdof, w, nv * 6);
+  const float* qvel = WPTR(d.qvel, w, nv);
+  if (t.valid && t.lane < 6) cvel[t.lane] = 0.0f;
+  sync();
+  for (int l = 1; l < m.nlevel; l++) {
+    const int beg = m.body_treeadr[l], end = level_end(m, l);
+    for (int k = beg + t.lane; k < end && t.valid; k += TEAM) {
+      const int b = m.body_tree[k];
+      const int pid = m.body_parentid[b];
+      int da = m.body_dofadr[b];
+      const int ja = m.body_jntadr[b], jn = m.body_jntnum[b];
+      S6 v = ld6(cvel + 6 * pid);
+      for (int j = ja; j < ja + jn; j++) {
+        const int jt = m.jnt_type[j];
+        if (jt == mjhJNT_FREE) {
+          for (int i = 0; i < 3; i++) {
+            const S6 c = ld6(cdof + 6 * (da + i));
+#pragma unroll
+            for (int q = 0; q < 6; q++) v.v[q] += c.v[q] * qvel[da + i];
+          }
+          for (int i = 3; i < 6; i++) st6(cdof_dot + 6 * (da + i), motion_cross(v, ld6(cdof + 6 * (da + i))));
+          for (int i = 3; i < 6; i++) {
+            const S6 c = ld6(cdof + 6 * (da + i));
+#pragma unroll
+            for (int q = 0; q < 6; q++) v.v[q] += c.v[q] * qvel[da + i];
+          }
+          da += 6;
+        } else if (jt == mjhJNT_BALL) {
+          for (int i = 0; i < 3; i++) st6(cdof_dot + 6 * (da + i), motion_cross(v, ld6(cdof + 6 * (da + i))));
+          for (int i = 0; i < 3; i++) {
+            const S6 c = ld6(cdof + 6 * (da + i));
+#pragma unroll
+            for (int q = 0; q < 6; q++) v.v[q] += c.v[q] * qvel[da + i];
+          }
+          da += 3;
+        } else {
+          const S6 c = ld6(cdof + 6 * da);
+          st6(cdof_dot + 6 * da, motion_cross(v, c));
+#pragma unroll
+          for (int q = 0; q < 6; q++) v.v[q] += c.v[q] * qvel[da];
+          da += 1;
+        }
+      }
+      st6(cvel + 6 * b, v);
+    }
+    sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// passive.py:27-108 passive
+template <int TEAM>
+MJH_DEV void passive(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+  const int w = t.world, nv = m.nv;
+  float* qfrc_passive = WPTR(d.qfrc_passive, w, nv);
+  if (m.opt.disableflags & mjhDSBL_PASSIVE) {
+    if (t.valid)
+      for (int i = t.lane; i < nv; i += TEAM) qfrc_passive[i] = 0.0f;
+    sync();
+    return;
+  }
+  float* spring = WPTR(d.qfrc_spring, w, nv);
+  float* damper = WPTR(d.qfrc_damper, w, nv);
+  const float* qpos = WPTR(d.qpos, w, m.nq);
+  const float* qvel = WPTR(d.qvel, w, nv);
+  if (t.valid)
+    for (int i = t.lane; i < nv; i += TEAM) spring[i] = 0.0f;
+  sync();
+  if (t.valid) {
+    for (int j = t.lane; j < m.njnt; j += TEAM) {
+      const float k = m.jnt_stiffness[j];
+      if (k == 0.0f) continue;
+      const int da = m.jnt_dofadr[j], qa = m.jnt_qposadr[j], jt = m.jnt_type[j];
+      if (jt == mjhJNT_FREE) {
+        for (int i = 0; i < 3; i++) spring[da + i] = -k * (qpos[qa + i] - m.qpos_spring[qa + i]);
+        const V3 dif = quat_sub(ld4(qpos + qa + 3), ld4(m.qpos_spring + qa + 3));
+        spring[da + 3] = -k * dif.x;
+        spring[da + 4] = -k * dif.y;
+        spring[da + 5] = -k * dif.z;
+      } else if (jt == mjhJNT_BALL) {
+        const V3 dif = quat_sub(ld4(qpos + qa), ld4(m.qpos_spring + qa));
+        spring[da + 0] = -k * dif.x;
+        spring[da + 1] = -k * dif.y;
+        spring[da + 2] = -k * dif.z;
+      } else {
+        spring[da] = -k * (qpos[qa] - m.qpos_spring[qa]);
+      }
+    }
+  }
+  sync();
+  if (t.valid)
+    for (int i = t.lane; i < nv; i += TEAM) {
+      const float f = -m.dof_damping[i] * qvel[i];
+      damper[i] = f;
+      qfrc_passive[i] = f + spring[i];
+    }
+  sync();
+}
+
+}  // namespace mjh

@@ -1,0 +1,65 @@
+"""Model builders for the benchmark and test workloads.
+
+* `chain(n)`      — n-link capsule hinge chain above a plane (the CPU-runnable
+                    plain-step workload: 3 links of 0.5 m x 0.05 m, 1 kg each,
+                    hinges about y, limits +-1.57 rad, first hinge at z = 1.5 m
+                    so the hanging chain's tip grazes the plane, Euler
+                    dt = 0.002, CG 100 iterations, tol 1e-8).
+* `humanoid()`    — the 27-dof humanoid (assets/humanoid.xml) with the
+                    headline-benchmark overrides: all 21 motor gears 100,
+                    implicitfast, Newton, iterations / ls_iterations 1 / 4
+                    (the reference benchmark's defaults, testspeed.py:42-59).
+"""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+
+from . import mjcf
+
+ASSETS = Path(__file__).resolve().parent / "assets"
+
+
+def chain_xml(n: int = 3, anchor_z: float = 1.5) -> str:
+  link = ('<joint name="hinge{i}" type="hinge" axis="0 1 0" range="-1.57 1.57" limited="true"/>'
+          '<geom name="link{i}" type="capsule" fromto="0 0 0 0 0 -0.5" size="0.05" mass="1"/>')
+  body = ""
+  for i in reversed(range(n)):
+    pos = f"0 0 {anchor_z}" if i == 0 else "0 0 -0.5"
+    body = f'<body name="body{i}" pos="{pos}">' + link.format(i=i) + body + "</body>"
+  return f"""
+<mujoco model="chain{n}">
+  <compiler angle="radian"/>
+  <option timestep="0.002" integrator="Euler" solver="CG" iterations="100" tolerance="1e-8"
+          ls_iterations="50"/>
+  <worldbody>
+    <geom name="floor" type="plane" size="0 0 0.05"/>
+    {body}
+  </worldbody>
+</mujoco>
+"""
+
+
+def chain(n: int = 3) -> mjcf.MjModel:
+  return mjcf.from_xml_string(chain_xml(n))
+
+
+def humanoid(gear: float = 100.0, iterations: int = 1, ls_iterations: int = 4,
+             integrator: int = 3, solver: int = 2) -> mjcf.MjModel:
+  m = mjcf.from_xml_path(str(ASSETS / "humanoid.xml"))
+  m.actuator_gear[:, 0] = gear
+  m.opt.integrator = integrator
+  m.opt.solver = solver
+  m.opt.iterations = iterations
+  m.opt.ls_iterations = ls_iterations
+  return m
+
+
+def chain_initial_state(nworld: int, seed: int = 0, nq: int = 3):
+  """qpos ~ U(-0.5, 0.5), qvel ~ N(0, 0.1) from numpy default_rng(seed)."""
+  rng = np.random.default_rng(seed)
+  qpos = rng.uniform(-0.5, 0.5, (nworld, nq))
+  qvel = rng.normal(0.0, 0.1, (nworld, nq))
+  return qpos, qvel

@@ -1,0 +1,26 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+  sys.path.insert(0, ROOT)
+TESTS = os.path.dirname(os.path.abspath(__file__))
+if TESTS not in sys.path:
+  sys.path.insert(0, TESTS)
+
+
+def pytest_configure(config):
+  config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+def pytest_collection_modifyitems(config, items):
+  import torch
+
+  if torch.cuda.is_available():
+    return
+  skip = pytest.mark.skip(reason="no HIP device")
+  for item in items:
+    if "gpu" in item.keywords:
+      item.add_marker(skip)
