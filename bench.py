@@ -1,0 +1,219 @@
+"""World-steps/s of the batched physics step on one MI355X.
+
+Workload (default): the 27-dof humanoid, B = 8192 worlds, all 21 motor gears
+100, ctrl ~ U(-1, 1) resampled every step on the device, Newton solver,
+implicitfast, dt = 0.005, FP32, from the "squat" keyframe.  Solver
+iterations / line-search iterations default to the model file's 10 / 20;
+--iterations 1 --ls_iterations 4 selects the reference benchmark's defaults
+(unstable for this model with implicitfast: worlds reach NaN within tens of
+steps).
+
+One step = the whole of mujoco_warp.step for all B worlds (forward
+kinematics, inertia, collision, constraint rows, solver, integration) plus
+the ctrl resample.  `value` is B * steps / wall time of the timed loop with
+all state resident in HBM; the fused step (one launch per step) is the
+headline, the plain per-stage step is reported beside it.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+  sys.path.insert(0, ROOT)
+
+METRIC = "world-steps/s, 27-DoF humanoid B=8192; fwd+adjoint world-steps/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+  print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(mjm, ncon_w: float, nefc_w: float) -> float:
+  """Bytes of Data fields one world-step must read or write, counted once in
+  FP32: reads R = nq + 3nv + nu + na + 6 nbody + 7 nmocap; writes
+  W = nq + 23nv + 57 nbody + 6 njnt + 12 ngeom + 12 nsite + 3nu + nu nv + 2na
+  + 2 nv^2 (dense qM and qLD); plus 30 words per contact and nv + 6 words per
+  constraint row, at the measured per-world means."""
+  R = mjm.nq + 3 * mjm.nv + mjm.nu + mjm.na + 6 * mjm.nbody + 7 * mjm.nmocap
+  W = (mjm.nq + 23 * mjm.nv + 57 * mjm.nbody + 6 * mjm.njnt + 12 * mjm.ngeom + 12 * mjm.nsite + 3 * mjm.nu
+       + mjm.nu * mjm.nv + 2 * mjm.na + 2 * mjm.nv * mjm.nv)
+  return 4.0 * (R + W + 30 * ncon_w + (mjm.nv + 6) * nefc_w)
+
+
+def setup(args, device="cuda"):
+  import numpy as np
+  import torch
+
+  import mjx_warp_amd as mjw
+  from mjx_warp_amd import models
+
+  mjm = models.humanoid(gear=100.0, iterations=args.iterations, ls_iterations=args.ls_iterations)
+  m = mjw.put_model(mjm, device=device)
+  B = args.batch
+  d = mjw.make_data(mjm, nworld=B, nconmax=64 * B, njmax=(4 * 64 + 21) * B, device=device)
+  d.qpos = torch.tensor(np.tile(mjm.key_qpos[0], (B, 1)), dtype=torch.float32, device=device)
+  return mjm, m, d
+
+
+def time_form(args, fused: bool):
+  import torch
+
+  import mjx_warp_amd as mjw
+
+  mjm, m, d = setup(args)
+  gen = torch.Generator(device="cuda")
+  gen.manual_seed(1)
+  stepfn = mjw.step_fused if fused else mjw.step
+  u = torch.empty_like(d.ctrl)
+
+  def one_step():
+    torch.rand(u.shape, generator=gen, device="cuda", out=u)
+    d.ctrl.copy_(u.mul_(2.0).sub_(1.0))
+    stepfn(m, d)
+
+  for _ in range(args.warmup):
+    one_step()
+  torch.cuda.synchronize()
+  stream = torch.cuda.current_stream()
+  ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+  ncon = torch.zeros((), dtype=torch.int64, device="cuda")
+  nefc = torch.zeros((), dtype=torch.int64, device="cuda")
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for k in range(args.steps):
+    torch.rand(u.shape, generator=gen, device="cuda", out=u)
+    d.ctrl.copy_(u.mul_(2.0).sub_(1.0))
+    ev[k][0].record(stream)
+    stepfn(m, d)
+    ev[k][1].record(stream)
+    ncon += d.ncon[0]
+    nefc += d.nefc[0]
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  step_ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+  B = args.batch
+  res = dict(wall=wall, ms_per_step=1000.0 * wall / args.steps, kernel_ms=step_ms,
+             value=B * args.steps / wall, ncon_w=float(ncon.item()) / args.steps / B,
+             nefc_w=float(nefc.item()) / args.steps / B,
+             finite=bool(torch.isfinite(d.qpos).all().item()))
+  return mjm, res
+
+
+def cpu_baseline(args, target_s: float = 15.0):
+  """The oracle (FP32 CPU restatement, OpenMP over worlds) on this host."""
+  import numpy as np
+  import torch
+
+  from oracle import oracle
+  from mjx_warp_amd import _native
+
+  sys.path.insert(0, os.path.join(ROOT, "tests"))
+  import helpers
+
+  cores = min(16, len(os.sched_getaffinity(0)))
+  B = min(args.batch, 1024)
+  mjm, m, d = setup(argparse.Namespace(**{**vars(args), "batch": B}), device="cpu")
+  hm = helpers.host_model(m, np.float32)
+  hd = helpers.host_data(m, d, np.float32)
+  rng = np.random.default_rng(1)
+  # probe
+  t0 = time.perf_counter()
+  hd.arrays["ctrl"][:] = rng.uniform(-1, 1, hd.arrays["ctrl"].shape)
+  oracle.run("step", hm, hd, cores)
+  probe = time.perf_counter() - t0
+  nstep = max(2, min(200, int(target_s / max(probe, 1e-4))))
+  t0 = time.perf_counter()
+  for _ in range(nstep):
+    hd.arrays["ctrl"][:] = rng.uniform(-1, 1, hd.arrays["ctrl"].shape)
+    oracle.run("step", hm, hd, cores)
+  dt = time.perf_counter() - t0
+  cpu = ""
+  try:
+    with open("/proc/cpuinfo") as f:
+      for line in f:
+        if line.startswith("model name"):
+          cpu = line.split(":", 1)[1].strip()
+          break
+  except OSError:
+    pass
+  return dict(value=B * nstep / dt, unit="world-steps/s", cores=cores, kind="port",
+              sample=f"{B} humanoid worlds x {nstep} steps of the oracle (FP32, OpenMP over worlds) "
+                     f"after 1 warm-up step, on {cpu or 'host'}")
+
+
+def main():
+  ap = argparse.ArgumentParser()
+  ap.add_argument("--gpus", type=int, default=1)
+  ap.add_argument("--steps", type=int, default=100)
+  ap.add_argument("--warmup", type=int, default=3)
+  ap.add_argument("--batch", type=int, default=8192)
+  ap.add_argument("--iterations", type=int, default=10)
+  ap.add_argument("--ls_iterations", type=int, default=20)
+  ap.add_argument("--no-plain", action="store_true", help="skip timing the plain per-stage step")
+  ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+  args = ap.parse_args()
+  if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+    print("bench.py measures one GPU: every box here has one MI355X and the step needs no "
+          "communication between GPUs (worlds are independent); run with --gpus 1.", file=sys.stderr)
+    sys.exit(2)
+
+  import torch
+
+  assert torch.cuda.is_available(), "bench.py needs the HIP device"
+  log(f"fused step: B={args.batch} steps={args.steps} warmup={args.warmup} "
+      f"iterations={args.iterations}/{args.ls_iterations}")
+  mjm, fused = time_form(args, fused=True)
+  log(f"fused: {fused['value']:.4g} world-steps/s, {fused['ms_per_step']:.3f} ms/step")
+  plain = None
+  if not args.no_plain:
+    _, plain = time_form(args, fused=False)
+    log(f"plain: {plain['value']:.4g} world-steps/s, {plain['ms_per_step']:.3f} ms/step")
+  bytes_ws = algorithmic_bytes(mjm, fused["ncon_w"], fused["nefc_w"])
+  achieved = bytes_ws * args.batch / (fused["kernel_ms"] * 1e-3) / 1e9
+  base = None
+  if not args.no_cpu:
+    log("cpu baseline ...")
+    base = cpu_baseline(args)
+    log(f"cpu: {base['value']:.4g} world-steps/s on {base['cores']} threads")
+  out = {
+    "metric": METRIC,
+    "value": fused["value"],
+    "unit": "world-steps/s",
+    "n_gpus": 1,
+    "steps": args.steps,
+    "warmup": args.warmup,
+    "ms_per_step": fused["ms_per_step"],
+    "higher_is_better": True,
+    "scaling": "weak",
+    "vs_baseline": None,
+    "dtype": "f32",
+    "data": "synthetic",
+    "config": {"workload": "humanoid 27-dof, fused step, ctrl ~ U(-1,1) resampled per step",
+               "nworld": args.batch, "solver": "newton", "iterations": args.iterations,
+               "ls_iterations": args.ls_iterations, "integrator": "implicitfast", "timestep": 0.005,
+               "gear": 100, "nconmax_per_world": 64, "njmax_per_world": 277},
+    "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                 "kernel": "step_fused (counter reset + fused step launch)",
+                 "algorithmic_bytes_per_world_step": bytes_ws, "kernel_ms": fused["kernel_ms"]},
+    "cpu_baseline": base,
+    "mean_ncon_per_world": fused["ncon_w"],
+    "mean_nefc_per_world": fused["nefc_w"],
+    "state_finite": fused["finite"],
+    "plain_step": None if plain is None else {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
+                                              "kernel_ms": plain["kernel_ms"]},
+  }
+  print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+  main()
