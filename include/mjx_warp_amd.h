@@ -328,12 +328,15 @@ typedef struct mjhOption {
   X(I, world_pair, (nworld, 2)) \
   X(I, world_con, (nworld, 2)) \
   X(I, world_efc, (nworld, 2)) \
+  X(I, world_flags, (nworld)) \
   X(I, overflow, (4))
-/* The last four arrays have no counterpart in types.py.  Each world's
+/* The last five arrays have no counterpart in types.py.  Each world's
  * broadphase pairs, contacts and constraint rows occupy one contiguous block
  * of the global pool, reserved with a single atomic per world; world_pair,
  * world_con and world_efc hold (first index, count written) of that block so
- * that later stages find a world's rows without scanning the pool.  overflow
+ * that later stages find a world's rows without scanning the pool.
+ * world_flags marks worlds whose rows exceed the LDS solver's row capacity;
+ * a tail launch of the same step solves and integrates them.  overflow
  * counts pairs / contacts / rows dropped because a pool was full (the
  * reference drops contacts silently and writes rows out of bounds,
  * collision_primitive.py:64-65, constraint.py:119,162).

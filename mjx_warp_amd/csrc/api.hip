@@ -3,10 +3,14 @@
 // Every stage kernel is launched with 64-lane workgroups (one wavefront),
 // each holding 64/TEAM worlds; the grid covers nworld.  The plain step
 // (mjh_step) launches one kernel per reference stage; the fused step
-// (mjh_step_fused) runs every stage of a world inside one launch.
+// (mjh_step_fused) runs every stage of a world inside one launch.  Both use
+// the LDS-resident solver; worlds whose constraint rows exceed its row
+// capacity are flagged and finished by a tail launch of the global-memory
+// solver (and, for the fused step, the integrator).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -15,6 +19,7 @@
 #include "forward.hpp"
 #include "smooth.hpp"
 #include "solver.hpp"
+#include "solver_lds.hpp"
 
 using namespace mjh;
 
@@ -29,113 +34,68 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kTeam = 64;
 
-// ---- stage functors ----
-struct StKinematics {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    kinematics(m, d, t);
-  }
+// launch-time tuning of the LDS solver: row capacity and staged J rows
+struct Tune {
+  int rmax;
+  int rj;
 };
-struct StComPos {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    com_pos(m, d, t);
-  }
-};
-struct StCrb {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    crb(m, d, t);
-  }
-};
-struct StFactorM {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
-    factor_m(m, d, l, t);
-  }
-};
-struct StRne {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    rne(m, d, t);
-  }
-};
-struct StTransmission {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    transmission(m, d, t);
-  }
-};
-struct StComVel {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    com_vel(m, d, t);
-  }
-};
-struct StPassive {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    passive(m, d, t);
-  }
-};
-struct StNxn {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    nxn_broadphase(m, d, t);
-  }
-};
-struct StSap {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
-    sap_broadphase(m, d, l, t);
-  }
-};
-struct StBroadphase {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
-    if (m.ngeom <= 100) nxn_broadphase(m, d, t);
-    else sap_broadphase(m, d, l, t);
-  }
-};
-struct StNarrowphase {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    narrowphase(m, d, t);
-  }
-};
-struct StMakeConstraint {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    make_constraint(m, d, t);
-  }
-};
-struct StFwdVelocity {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    fwd_velocity(m, d, t);
-  }
-};
-struct StFwdActuation {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    fwd_actuation(m, d, t);
-  }
-};
-struct StFwdAcceleration {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
-    fwd_acceleration(m, d, l, t);
-  }
-};
-struct StSolve {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
+
+#define STAGE(NAME, CALL)                                                                              \
+  struct NAME {                                                                                        \
+    template <int T>                                                                                   \
+    MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t,          \
+                            const Tune& tune) const {                                                  \
+      (void)l;                                                                                         \
+      (void)tune;                                                                                      \
+      CALL;                                                                                            \
+    }                                                                                                  \
+  };
+
+STAGE(StKinematics, kinematics(m, d, t))
+STAGE(StComPos, com_pos(m, d, t))
+STAGE(StCrb, crb(m, d, t))
+STAGE(StFactorM, factor_m(m, d, l, t))
+STAGE(StRne, rne(m, d, t))
+STAGE(StTransmission, transmission(m, d, t))
+STAGE(StComVel, com_vel(m, d, t))
+STAGE(StPassive, passive(m, d, t))
+STAGE(StNxn, nxn_broadphase(m, d, t))
+STAGE(StSap, sap_broadphase(m, d, l, t))
+STAGE(StBroadphase, if (m.ngeom <= 100) nxn_broadphase(m, d, t); else sap_broadphase(m, d, l, t))
+STAGE(StNarrowphase, narrowphase(m, d, t))
+STAGE(StMakeConstraint, make_constraint(m, d, t))
+STAGE(StFwdVelocity, fwd_velocity(m, d, t))
+STAGE(StFwdActuation, fwd_actuation(m, d, t))
+STAGE(StFwdAcceleration, fwd_acceleration(m, d, l, t))
+STAGE(StEuler, euler(m, d, l, t))
+STAGE(StImplicit, implicit(m, d, l, t))
+// forward.py:676-677 (njmax == 0)
+STAGE(StQaccSmooth, if (t.valid) for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] =
+                        d.qacc_smooth[t.world * m.nv + i])
+// LDS solver; worlds over the row capacity are flagged for the tail
+STAGE(StSolve, if (!solve_lds(m, d, l, tune.rmax, tune.rj, t) && t.lane == 0) d.world_flags[t.world] |=
+               kFlagDeferred)
+// tail: flagged worlds get the global-memory solver (and, when `integrate`
+// is set, the integrator), then the flag is cleared
+template <bool integrate>
+struct StTail {
+  template <int T>
+  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune&) const {
+    if (!t.valid || !(d.world_flags[t.world] & kFlagDeferred)) return;
     solve(m, d, l, t);
-  }
-};
-struct StQaccSmooth {  // forward.py:676-677 (njmax == 0)
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float*, const Team<T>& t) const {
-    if (!t.valid) return;
-    for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] = d.qacc_smooth[t.world * m.nv + i];
-  }
-};
-struct StEuler {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
-    euler(m, d, l, t);
-  }
-};
-struct StImplicit {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
-    implicit(m, d, l, t);
+    sync();
+    if (integrate) {
+      if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
+      else implicit(m, d, l, t);
+    }
+    if (t.lane == 0) d.world_flags[t.world] &= ~kFlagDeferred;
   }
 };
 
 // The whole step of one world (forward.py:682-695 / 664-679).
 struct StFusedStep {
-  template <int T> MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) const {
+  template <int T>
+  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
     kinematics(m, d, t);
     com_pos(m, d, t);
     crb(m, d, t);
@@ -152,8 +112,9 @@ struct StFusedStep {
       if (t.valid)
         for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] = d.qacc_smooth[t.world * m.nv + i];
       sync();
-    } else {
-      solve(m, d, l, t);
+    } else if (!solve_lds(m, d, l, tune.rmax, tune.rj, t)) {
+      if (t.lane == 0) d.world_flags[t.world] |= kFlagDeferred;
+      return;  // solved and integrated by the tail launch
     }
     sync();
     if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
@@ -162,10 +123,10 @@ struct StFusedStep {
 };
 
 template <int TEAM, class S>
-__global__ void __launch_bounds__(64) k_stage(mjhModel m, mjhData d, int lds_per_world) {
+__global__ void __launch_bounds__(64) k_stage(mjhModel m, mjhData d, int lds_per_world, Tune tune) {
   extern __shared__ float smem[];
   Team<TEAM> t(d.nworld);
-  S()(m, d, smem + t.slot * lds_per_world, t);
+  S()(m, d, smem + t.slot * lds_per_world, t, tune);
 }
 
 // counter reset without memset nodes (capture-safe):
@@ -207,8 +168,27 @@ int lds_sap(const mjhModel& m) {
   while (np < m.ngeom) np <<= 1;
   return 2 * np + 4 + 4 * m.ngeom;
 }
+
+// Row capacity of the LDS solver (rows of one world) and how many of them
+// get their Jacobian rows staged in LDS; MJH_SOLVER_RMAX / MJH_SOLVER_RJ
+// override the defaults.
+Tune tune_for(const mjhModel& m) {
+  Tune t{64, 32};
+  if (const char* e = std::getenv("MJH_SOLVER_RMAX")) t.rmax = std::atoi(e);
+  if (const char* e = std::getenv("MJH_SOLVER_RJ")) t.rj = std::atoi(e);
+  if (t.rmax < 0) t.rmax = 0;
+  if (t.rj > t.rmax) t.rj = t.rmax;
+  if (t.rj < 0) t.rj = 0;
+  return t;
+}
+int lds_solve(const mjhModel& m) {
+  const Tune t = tune_for(m);
+  const int a = lds_solver_floats(m.nv, t.rmax, t.rj), b = lds_linalg(m);
+  return a > b ? a : b;
+}
 int lds_step(const mjhModel& m) {
-  const int a = lds_linalg(m), b = m.ngeom > 100 ? lds_sap(m) : 0;
+  int a = lds_solve(m);
+  const int b = m.ngeom > 100 ? lds_sap(m) : 0;
   return a > b ? a : b;
 }
 
@@ -224,7 +204,10 @@ int launch(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, const c
   constexpr int wpb = 64 / kTeam;
   const int grid = (d.nworld + wpb - 1) / wpb;
   const size_t bytes = (size_t)ldsw * wpb * sizeof(float);
-  hipLaunchKernelGGL((k_stage<kTeam, S>), dim3(grid), dim3(64), bytes, s, m, d, ldsw);
+  if (bytes > 65536)
+    return fail(MJH_ERR_CAPACITY, std::string(name) + ": model too large for the dense LDS kernels (nv = " +
+                                      std::to_string(m.nv) + ")");
+  hipLaunchKernelGGL((k_stage<kTeam, S>), dim3(grid), dim3(64), bytes, s, m, d, ldsw, tune_for(m));
   return check_launch(name);
 }
 
@@ -233,15 +216,20 @@ int reset(const mjhData& d, hipStream_t s, int which) {
   return check_launch("reset");
 }
 
-#define TRY(x)                 \
-  do {                         \
-    int _rc = (x);             \
+#define TRY(x)                     \
+  do {                             \
+    int _rc = (x);                 \
     if (_rc != MJH_OK) return _rc; \
   } while (0)
 
 int do_collision(const mjhModel& m, const mjhData& d, hipStream_t s) {
   TRY(launch<StBroadphase>(m, d, s, m.ngeom > 100 ? lds_sap(m) : 0, "broadphase"));
   return launch<StNarrowphase>(m, d, s, 0, "narrowphase");
+}
+
+int do_solve(const mjhModel& m, const mjhData& d, hipStream_t s) {
+  TRY(launch<StSolve>(m, d, s, lds_solve(m), "solve"));
+  return launch<StTail<false>>(m, d, s, lds_linalg(m), "solve_tail");
 }
 
 int do_fwd_position(const mjhModel& m, const mjhData& d, hipStream_t s) {
@@ -264,7 +252,7 @@ int do_forward(const mjhModel& m, const mjhData& d, hipStream_t s) {
   TRY(launch<StFwdActuation>(m, d, s, 0, "fwd_actuation"));
   TRY(launch<StFwdAcceleration>(m, d, s, lds_linalg(m), "fwd_acceleration"));
   if (d.njmax == 0) return launch<StQaccSmooth>(m, d, s, 0, "qacc_smooth");
-  return launch<StSolve>(m, d, s, lds_linalg(m), "solve");
+  return do_solve(m, d, s);
 }
 
 int do_integrate(const mjhModel& m, const mjhData& d, hipStream_t s) {
@@ -277,7 +265,8 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
   if (m.opt.integrator != mjhINT_EULER && m.opt.integrator != mjhINT_IMPLICITFAST)
     return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
   TRY(reset(d, s, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3));
-  return launch<StFusedStep>(m, d, s, lds_step(m), "step_fused");
+  TRY(launch<StFusedStep>(m, d, s, lds_step(m), "step_fused"));
+  return launch<StTail<true>>(m, d, s, lds_linalg(m), "step_tail");
 }
 
 }  // namespace
@@ -297,14 +286,12 @@ int mjh_check(const mjhModel* m, const mjhData* d) {
   if (m->nlevel < 1 || m->nbody < 1) return fail(MJH_ERR_INVALID, "model has no bodies");
   if (d->nworld < 0 || d->nconmax < 0 || d->njmax < 0) return fail(MJH_ERR_INVALID, "negative sizes");
   if (d->qM_rows != m->nv || d->qM_cols != m->nv) return fail(MJH_ERR_INVALID, "qM must be dense [nworld, nv, nv]");
-  const size_t bytes = (size_t)lds_step(*m) * sizeof(float);
-  if (bytes > 65536) return fail(MJH_ERR_CAPACITY, "model too large for the dense LDS kernels (nv too large)");
   return MJH_OK;
 }
 
-#define STAGE_ENTRY(fname, S, ldsexpr)                                            \
-  int fname(const mjhModel* m, const mjhData* d, mjhStream stream) {              \
-    return launch<S>(*m, *d, (hipStream_t)stream, (ldsexpr), #fname);             \
+#define STAGE_ENTRY(fname, S, ldsexpr)                                \
+  int fname(const mjhModel* m, const mjhData* d, mjhStream stream) {  \
+    return launch<S>(*m, *d, (hipStream_t)stream, (ldsexpr), #fname); \
   }
 
 STAGE_ENTRY(mjh_kinematics, StKinematics, 0)
@@ -318,8 +305,10 @@ STAGE_ENTRY(mjh_passive, StPassive, 0)
 STAGE_ENTRY(mjh_fwd_velocity, StFwdVelocity, 0)
 STAGE_ENTRY(mjh_fwd_actuation, StFwdActuation, 0)
 STAGE_ENTRY(mjh_fwd_acceleration, StFwdAcceleration, lds_linalg(*m))
-STAGE_ENTRY(mjh_solve, StSolve, lds_linalg(*m))
 
+int mjh_solve(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  return do_solve(*m, *d, (hipStream_t)stream);
+}
 int mjh_nxn_broadphase(const mjhModel* m, const mjhData* d, mjhStream stream) {
   TRY(reset(*d, (hipStream_t)stream, 1));
   return launch<StNxn>(*m, *d, (hipStream_t)stream, 0, "nxn_broadphase");
@@ -368,6 +357,7 @@ int mjh_rollout(const mjhModel* m, const mjhData* d, int nstep, int fused, mjhSt
 int mjh_solve_m(const mjhModel* m, const mjhData* d, float* x, const float* y, mjhStream stream) {
   if (d->nworld <= 0) return MJH_OK;
   const int ldsw = lds_linalg(*m);
+  if ((size_t)ldsw * sizeof(float) > 65536) return fail(MJH_ERR_CAPACITY, "solve_m: nv too large");
   hipLaunchKernelGGL(k_solve_m, dim3(d->nworld), dim3(64), ldsw * sizeof(float), (hipStream_t)stream, *m, *d, x, y,
                      ldsw);
   return check_launch("solve_m");

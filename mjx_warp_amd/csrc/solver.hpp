@@ -316,11 +316,13 @@ struct SolverWorld {
     sync();
   }
 
-  MJH_DEV void search_update(float beta) {
+  MJH_DEV void search_update(float beta, bool with_previous) {
+    // with_previous is false for the first direction (solver.py:50-55),
+    // which does not read the previous search vector
     float sd = 0.0f;
     for (int i = t.lane; i < nv; i += TEAM) {
       float s = -1.0f * Mgrad[i];
-      if (m.opt.solver == mjhSOL_CG) s += beta * search[i];
+      if (with_previous && m.opt.solver == mjhSOL_CG) s += beta * search[i];
       search[i] = s;
       sd += s * s;
     }
@@ -362,7 +364,7 @@ MJH_DEV void solve(const mjhModel& m, const mjhData& d, float* lds, const Team<T
   sync();
   s.update_constraint();
   s.update_gradient();
-  s.search_update(0.0f);
+  s.search_update(0.0f, false);
   const int iters = m.opt.iterations;
   bool done = false;
   for (int it = 0; it < iters; it++) {
@@ -395,7 +397,7 @@ MJH_DEV void solve(const mjhModel& m, const mjhData& d, float* lds, const Team<T
         d.efc_beta[w] = beta;
       }
     }
-    s.search_update(beta);
+    s.search_update(beta, true);
     const float scale = m.meaninertia * (float)max(1, nv);
     const float improvement = (d.efc_prev_cost[w] - d.efc_cost[w]) / scale;
     const float gradient = sqrtf(d.efc_grad_dot[w]) / scale;

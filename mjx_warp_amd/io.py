@@ -222,15 +222,24 @@ def put_model(mjm, device=None) -> types.Model:
   m.stat.meaninertia = float(mjm.stat.meaninertia)
 
   _, arrays = _native.layout()["model"]
-  sizes = _native.model_sizes(m)
-  for kind, name, dims in arrays:
-    src = tables[name] if name in tables else getattr(mjm, name)
-    shape = _native.shape_of(dims, sizes)
-    arr = np.asarray(src).reshape(shape)
-    setattr(m, name, _tensor(arr, kind, device))
+  host = model_arrays(mjm, _native.model_sizes(m), tables)
+  for kind, name, _ in arrays:
+    setattr(m, name, _tensor(host[name], kind, device))
   if m.opt.is_sparse:
     raise NotImplementedError("sparse mass matrix (nv >= 60 or jacobian=sparse) is not implemented")
   return m
+
+
+def model_arrays(mjm, sizes: dict, tables: dict | None = None) -> dict:
+  """Every Model array of the C ABI as a numpy array in the MjModel's own
+  precision, shaped by the field list (the input of put_model)."""
+  tables = tables if tables is not None else model_tables(mjm)
+  _, arrays = _native.layout()["model"]
+  out = {}
+  for _, name, dims in arrays:
+    src = tables[name] if name in tables else getattr(mjm, name)
+    out[name] = np.asarray(src).reshape(_native.shape_of(dims, sizes))
+  return out
 
 
 def _alloc_data(mjm, nworld, nconmax, njmax, device) -> types.Data:

@@ -431,8 +431,9 @@ class _Compiler:
       if t in (3, 5):
         size[1] = half
       elif t in (4, 6):
-        size[0] = size[0]
-        size[1] = size[0] if len(g.get("size", [])) < 2 else size[1]
+        # the two sizes orthogonal to the axis; one given value serves both
+        if size[1] == 0.0:
+          size[1] = size[0]
         size[2] = half
     else:
       pos = np.array(g.get("pos", [0.0, 0.0, 0.0]), dtype=np.float64)

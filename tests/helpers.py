@@ -53,3 +53,40 @@ def world_blocks(pool_adr_num, nworld):
 def gather_world_rows(arr, blocks):
   """Concatenates each world's block of a pool array in world order."""
   return np.concatenate([arr[b:b + n] for b, n in blocks], axis=0) if blocks else arr[:0]
+
+
+def load(xml_or_mjm, nworld=1, keyframe=None, device="cpu", nconmax=None, njmax=None, real=np.float64):
+  """(mjm, m, d, host model, host data) for an MJCF string or compiled model."""
+  from mjx_warp_amd import mjcf
+
+  mjm = mjcf.from_xml_string(xml_or_mjm) if isinstance(xml_or_mjm, str) else xml_or_mjm
+  m = io.put_model(mjm, device=device)
+  d = io.make_data(mjm, nworld=nworld, nconmax=nconmax or max(64, 16 * nworld),
+                   njmax=njmax or max(256, 64 * nworld), device=device)
+  hm = host_model_exact(mjm, m, real)
+  hd = host_data(m, d, real)
+  hd.arrays["qpos"][:] = mjm.qpos0[None]
+  if keyframe is not None:
+    set_field(d, "qpos", np.tile(mjm.key_qpos[keyframe], (nworld, 1)))
+    set_field(d, "qvel", np.tile(mjm.key_qvel[keyframe], (nworld, 1)))
+    hd.arrays["qpos"][:] = mjm.key_qpos[keyframe][None]
+    hd.arrays["qvel"][:] = mjm.key_qvel[keyframe][None]
+  return mjm, m, d, hm, hd
+
+
+def host_model_exact(mjm, m, real=np.float64) -> oracle.HostModel:
+  """Oracle model whose arrays come straight from the compiled MjModel (no
+  float32 round trip), so float64 checks see the model's exact values."""
+  base = host_model(m, real)
+  arrays = io.model_arrays(mjm, base.sizes)
+  o = mjm.opt
+  opt = dict(base.opt, timestep=o.timestep, impratio=o.impratio, tolerance=o.tolerance,
+             ls_tolerance=o.ls_tolerance, gravity=np.asarray(o.gravity, np.float64))
+  return oracle.HostModel(base.sizes, opt, float(mjm.stat.meaninertia), arrays, real)
+
+
+def pairs_of(hd, world=0):
+  """Sorted (g1, g2) pairs of one world from the broadphase pool."""
+  adr, n = hd["world_pair"].reshape(-1, 2)[world]
+  p = hd["collision_pair"].reshape(-1, 2)[adr:adr + n]
+  return sorted(tuple(sorted((int(a), int(b)))) for a, b in p)
