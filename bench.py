@@ -150,6 +150,20 @@ def cpu_baseline(args, target_s: float = 15.0):
                      f"after 1 warm-up step, on {cpu or 'host'}")
 
 
+def copy_probe(mib: int, reps: int = 5):
+  """A plain device-to-device copy of known size: the reference point that
+  scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+  import torch
+
+  n = mib * (1 << 20) // 4
+  src = torch.ones(n, dtype=torch.float32, device="cuda")
+  dst = torch.empty_like(src)
+  for _ in range(reps):
+    dst.copy_(src)
+  torch.cuda.synchronize()
+  log(f"copy probe: {reps} copies of {mib} MiB ({4 * n} bytes read and written each)")
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -160,7 +174,12 @@ def main():
   ap.add_argument("--ls_iterations", type=int, default=20)
   ap.add_argument("--no-plain", action="store_true", help="skip timing the plain per-stage step")
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+  ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                  help="only run a plain copy of MIB MiB (counter calibration) and exit")
   args = ap.parse_args()
+  if args.copy_probe:
+    copy_probe(args.copy_probe)
+    return
   if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
     print("bench.py measures one GPU: every box here has one MI355X and the step needs no "
           "communication between GPUs (worlds are independent); run with --gpus 1.", file=sys.stderr)

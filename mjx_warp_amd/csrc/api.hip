@@ -20,6 +20,7 @@
 #include "smooth.hpp"
 #include "solver.hpp"
 #include "solver_lds.hpp"
+#include "solver_reg.hpp"
 
 using namespace mjh;
 
@@ -72,9 +73,14 @@ STAGE(StImplicit, implicit(m, d, l, t))
 // forward.py:676-677 (njmax == 0)
 STAGE(StQaccSmooth, if (t.valid) for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] =
                         d.qacc_smooth[t.world * m.nv + i])
-// LDS solver; worlds over the row capacity are flagged for the tail
-STAGE(StSolve, if (!solve_lds(m, d, l, tune.rmax, tune.rj, t) && t.lane == 0) d.world_flags[t.world] |=
-               kFlagDeferred)
+// Solver of one world: the register solver on 32-lane teams (nv <= 32),
+// else the LDS solver; false when the world is left for the tail.
+template <int T>
+MJH_DEV bool solve_team(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) {
+  if constexpr (T == kRegTeam) return solve_reg(m, d, l, tune.rj, t);
+  else return solve_lds(m, d, l, tune.rmax, tune.rj, t);
+}
+STAGE(StSolveTeam, if (!solve_team(m, d, l, t, tune) && t.lane == 0) d.world_flags[t.world] |= kFlagDeferred)
 // tail: flagged worlds get the global-memory solver (and, when `integrate`
 // is set, the integrator), then the flag is cleared
 template <bool integrate>
@@ -112,7 +118,45 @@ struct StFusedStep {
       if (t.valid)
         for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] = d.qacc_smooth[t.world * m.nv + i];
       sync();
-    } else if (!solve_lds(m, d, l, tune.rmax, tune.rj, t)) {
+    } else if (!solve_team(m, d, l, t, tune)) {
+      if (t.lane == 0) d.world_flags[t.world] |= kFlagDeferred;
+      return;  // solved and integrated by the tail launch
+    }
+    sync();
+    if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
+    else implicit(m, d, l, t);
+  }
+};
+
+// The fused step in two launches, for 32-lane teams: the forward stages up
+// to qacc_smooth, then the solver and the integrator.  One kernel holding
+// both would need the register and SGPR budget of the largest stage for
+// the whole step.
+struct StFusedForward {
+  template <int T>
+  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune&) const {
+    kinematics(m, d, t);
+    com_pos(m, d, t);
+    crb(m, d, t);
+    factor_m(m, d, l, t);
+    sync();
+    collision(m, d, l, t);
+    make_constraint(m, d, t);
+    sync();
+    transmission(m, d, t);
+    fwd_velocity(m, d, t);
+    fwd_actuation(m, d, t);
+    fwd_acceleration(m, d, l, t);
+  }
+};
+struct StFusedSolveIntegrate {
+  template <int T>
+  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
+    if (d.njmax == 0) {
+      if (t.valid)
+        for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] = d.qacc_smooth[t.world * m.nv + i];
+      sync();
+    } else if (!solve_team(m, d, l, t, tune)) {
       if (t.lane == 0) d.world_flags[t.world] |= kFlagDeferred;
       return;  // solved and integrated by the tail launch
     }
@@ -124,6 +168,7 @@ struct StFusedStep {
 
 template <int TEAM, class S>
 __global__ void __launch_bounds__(64) k_stage(mjhModel m, mjhData d, int lds_per_world, Tune tune) {
+  static_assert(64 % TEAM == 0, "teams tile the wavefront");
   extern __shared__ float smem[];
   Team<TEAM> t(d.nworld);
   S()(m, d, smem + t.slot * lds_per_world, t, tune);
@@ -192,22 +237,44 @@ int lds_step(const mjhModel& m) {
   return a > b ? a : b;
 }
 
+// MJH_SOLVER_PATH=reg runs worlds of models with nv <= 32 on 32-lane teams
+// (two per wavefront) with the register solver; by default every model
+// uses 64-lane teams and the LDS solver.
+bool reg_path(const mjhModel& m) {
+  static const bool use_reg = [] {
+    const char* e = std::getenv("MJH_SOLVER_PATH");
+    return e && std::strcmp(e, "reg") == 0;
+  }();
+  return m.nv <= kRegNvMax && use_reg;
+}
+// per-world LDS of the 32-lane kernels, a multiple of 4 floats so every
+// world's broadcast buffers are float4-aligned
+int lds_solve_reg(const mjhModel& m) {
+  const int a = lds_reg_floats(tune_for(m).rj), b = lds_linalg(m);
+  return ((a > b ? a : b) + 3) & ~3;
+}
+int lds_step_reg(const mjhModel& m) {
+  const int a = lds_solve_reg(m);
+  const int b = m.ngeom > 100 ? lds_sap(m) : 0;
+  return ((a > b ? a : b) + 3) & ~3;
+}
+
 int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(MJH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
   return MJH_OK;
 }
 
-template <class S>
+template <class S, int TEAM = kTeam>
 int launch(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, const char* name) {
   if (d.nworld <= 0) return MJH_OK;
-  constexpr int wpb = 64 / kTeam;
+  constexpr int wpb = 64 / TEAM;
   const int grid = (d.nworld + wpb - 1) / wpb;
   const size_t bytes = (size_t)ldsw * wpb * sizeof(float);
   if (bytes > 65536)
     return fail(MJH_ERR_CAPACITY, std::string(name) + ": model too large for the dense LDS kernels (nv = " +
                                       std::to_string(m.nv) + ")");
-  hipLaunchKernelGGL((k_stage<kTeam, S>), dim3(grid), dim3(64), bytes, s, m, d, ldsw, tune_for(m));
+  hipLaunchKernelGGL((k_stage<TEAM, S>), dim3(grid), dim3(64), bytes, s, m, d, ldsw, tune_for(m));
   return check_launch(name);
 }
 
@@ -228,7 +295,8 @@ int do_collision(const mjhModel& m, const mjhData& d, hipStream_t s) {
 }
 
 int do_solve(const mjhModel& m, const mjhData& d, hipStream_t s) {
-  TRY(launch<StSolve>(m, d, s, lds_solve(m), "solve"));
+  if (reg_path(m)) TRY((launch<StSolveTeam, kRegTeam>(m, d, s, lds_solve_reg(m), "solve")));
+  else TRY(launch<StSolveTeam>(m, d, s, lds_solve(m), "solve"));
   return launch<StTail<false>>(m, d, s, lds_linalg(m), "solve_tail");
 }
 
@@ -265,7 +333,12 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
   if (m.opt.integrator != mjhINT_EULER && m.opt.integrator != mjhINT_IMPLICITFAST)
     return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
   TRY(reset(d, s, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3));
-  TRY(launch<StFusedStep>(m, d, s, lds_step(m), "step_fused"));
+  if (reg_path(m)) {
+    TRY((launch<StFusedForward, kRegTeam>(m, d, s, lds_step_reg(m), "step_fused_forward")));
+    TRY((launch<StFusedSolveIntegrate, kRegTeam>(m, d, s, lds_solve_reg(m), "step_fused_solve")));
+  } else {
+    TRY(launch<StFusedStep>(m, d, s, lds_step(m), "step_fused"));
+  }
   return launch<StTail<true>>(m, d, s, lds_linalg(m), "step_tail");
 }
 
