@@ -372,3 +372,169 @@ def test_cg_and_newton_converge_to_the_same_acceleration():
     oracle.run("forward", hm, hd)
     out.append(hd["qacc"][0].copy())
   np.testing.assert_allclose(out[0], out[1], rtol=1e-6, atol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# actuation, passive forces and integrators against physical definitions
+def test_joint_motor_transmission_and_force():
+  """A motor on a hinge with gear g: length g q, moment g on the joint's dof,
+  velocity = d(length)/dt, torque g ctrl (ctrl clamped to ctrlrange)."""
+  mjm, q = _humanoid_state(4)
+  rng = np.random.default_rng(4)
+  v = rng.normal(0, 1.0, mjm.nv)
+  u = rng.uniform(-1.5, 1.5, mjm.nu)
+  mjm, m, d, hm, hd = helpers.load(mjm)
+  hd.arrays["qpos"][0], hd.arrays["qvel"][0], hd.arrays["ctrl"][0] = q, v, u
+  for s in ("kinematics", "com_pos", "crb", "transmission", "fwd_velocity", "fwd_actuation"):
+    oracle.run(s, hm, hd)
+  length = hd["actuator_length"][0]
+  moment = hd["actuator_moment"][0].reshape(mjm.nu, mjm.nv)
+  expect_f = np.zeros(mjm.nv)
+  for a in range(mjm.nu):
+    j = mjm.actuator_trnid[a, 0]
+    g = mjm.actuator_gear[a, 0]
+    dof, qa = mjm.jnt_dofadr[j], mjm.jnt_qposadr[j]
+    np.testing.assert_allclose(length[a], g * q[qa], rtol=1e-12)
+    row = np.zeros(mjm.nv)
+    row[dof] = g
+    np.testing.assert_array_equal(moment[a], row)
+    expect_f[dof] += g * np.clip(u[a], *mjm.actuator_ctrlrange[a])
+  # actuator velocity is the time derivative of the length
+  eps = 1e-7
+  mjm2, _, _, hm2, hd2 = helpers.load(mjm)
+  hd2.arrays["qpos"][0] = q
+  hd2.arrays["qpos"][0][7:] += eps * v[6:]
+  for s in ("kinematics", "com_pos", "transmission"):
+    oracle.run(s, hm2, hd2)
+  np.testing.assert_allclose(hd["actuator_velocity"][0], (hd2["actuator_length"][0] - length) / eps, rtol=1e-5, atol=1e-5)
+  np.testing.assert_allclose(hd["qfrc_actuator"][0], expect_f, rtol=1e-12, atol=1e-12)
+
+
+PASSIVE_XML = """<mujoco><compiler angle="radian"/><option gravity="0 0 0"/><worldbody>
+  <body><joint name="h" type="hinge" axis="0 1 0" stiffness="3" springref="0.2" damping="0.5"/>
+    <geom type="sphere" size="0.1" mass="1"/>
+    <body pos="0 0 -0.3"><joint name="b" type="ball" stiffness="2" damping="0.7"/>
+      <geom type="sphere" size="0.1" mass="1"/></body></body></worldbody></mujoco>"""
+
+
+def test_spring_and_damper_forces():
+  """Hinge: -k (q - springref) and -b qdot.  Ball: the spring torque is
+  -k theta axis for a rotation of theta about axis (rest = identity)."""
+  mjm, m, d, hm, hd = helpers.load(PASSIVE_XML)
+  axis = np.array([0.3, -0.5, 0.8]) / np.linalg.norm([0.3, -0.5, 0.8])
+  theta = 0.7
+  qball = np.concatenate([[np.cos(theta / 2)], np.sin(theta / 2) * axis])
+  hd.arrays["qpos"][0] = np.concatenate([[0.9], qball])
+  v = np.array([1.3, 0.2, -0.4, 0.5])
+  hd.arrays["qvel"][0] = v
+  oracle.run("passive", hm, hd)
+  np.testing.assert_allclose(hd["qfrc_spring"][0], np.concatenate([[-3 * (0.9 - 0.2)], -2 * theta * axis]), atol=1e-12)
+  np.testing.assert_allclose(hd["qfrc_damper"][0], -np.array([0.5, 0.7, 0.7, 0.7]) * v, atol=1e-12)
+  np.testing.assert_allclose(hd["qfrc_passive"][0], hd["qfrc_spring"][0] + hd["qfrc_damper"][0], atol=1e-12)
+
+
+@pytest.mark.parametrize("integrator", ["Euler", "implicitfast"])
+def test_damped_slide_is_backward_euler_in_the_damping(integrator):
+  """A slide joint of mass m and damping b, no other force: both Euler (with
+  implicit damping) and implicitfast step qdot' = qdot m / (m + dt b)."""
+  xml = f"""<mujoco><option gravity="0 0 0" timestep="0.01" integrator="{integrator}"/><worldbody>
+    <body><joint type="slide" axis="1 0 0" damping="3"/><geom type="sphere" size="0.1" mass="2"/></body>
+    </worldbody></mujoco>"""
+  mjm, m, d, hm, hd = helpers.load(xml)
+  hd.arrays["qpos"][0] = [0.25]
+  hd.arrays["qvel"][0] = [1.5]
+  oracle.run("step", hm, hd)
+  dt, mass, b = 0.01, 2.0, 3.0
+  v1 = 1.5 * mass / (mass + dt * b)
+  np.testing.assert_allclose(hd["qvel"][0], [v1], rtol=1e-12)
+  np.testing.assert_allclose(hd["qpos"][0], [0.25 + dt * v1], rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# primitive contacts: distance, position midway between the surfaces, normal
+def _contacts(xml, qpos=None):
+  mjm, m, d, hm, hd = helpers.load(xml)
+  if qpos is not None:
+    hd.arrays["qpos"][0] = qpos
+  oracle.run("kinematics", hm, hd)
+  oracle.run("collision", hm, hd)
+  n = int(hd["ncon"][0])
+  return (hd["contact_dist"][:n], hd["contact_pos"][:n], hd["contact_frame"][:n].reshape(n, 3, 3),
+          hd["contact_geom"][:n])
+
+
+def test_sphere_sphere_contact_closed_form():
+  c1, c2, r1, r2 = np.array([0.1, 0.0, 1.0]), np.array([0.25, 0.1, 1.05]), 0.1, 0.15
+  xml = f"""<mujoco><option gravity="0 0 0"/><worldbody>
+    <body pos="{c1[0]} {c1[1]} {c1[2]}"><freejoint/><geom type="sphere" size="{r1}"/></body>
+    <body pos="{c2[0]} {c2[1]} {c2[2]}"><freejoint/><geom type="sphere" size="{r2}"/></body></worldbody></mujoco>"""
+  dist, pos, frame, geom = _contacts(xml)
+  assert len(dist) == 1 and list(geom[0]) == [0, 1]
+  n = (c2 - c1) / np.linalg.norm(c2 - c1)
+  dd = np.linalg.norm(c2 - c1) - r1 - r2
+  np.testing.assert_allclose(dist[0], dd, atol=1e-12)
+  np.testing.assert_allclose(frame[0][0], n, atol=1e-12)
+  np.testing.assert_allclose(pos[0], c1 + n * (r1 + 0.5 * dd), atol=1e-12)
+  np.testing.assert_allclose(frame[0] @ frame[0].T, np.eye(3), atol=1e-12)
+
+
+def test_plane_capsule_contacts_at_both_ends():
+  h, half, r = 0.08, 0.2, 0.1
+  xml = f"""<mujoco><worldbody><geom type="plane" size="5 5 .1"/>
+    <body pos="0.1 0.2 {h}"><freejoint/><geom type="capsule" fromto="{-half} 0 0 {half} 0 0" size="{r}"/></body>
+    </worldbody></mujoco>"""
+  dist, pos, frame, geom = _contacts(xml)
+  assert len(dist) == 2
+  np.testing.assert_allclose(dist, [h - r, h - r], atol=1e-12)
+  ends = sorted(pos[:, 0])
+  np.testing.assert_allclose(ends, [0.1 - half, 0.1 + half], atol=1e-12)
+  np.testing.assert_allclose(pos[:, 2], h - r - 0.5 * (h - r), atol=1e-12)
+  for f in frame:
+    np.testing.assert_allclose(f[0], [0, 0, 1], atol=1e-12)
+    np.testing.assert_allclose(abs(f[1] @ [1, 0, 0]), 1.0, atol=1e-12)  # tangent along the capsule axis
+
+
+def test_plane_box_corner_contacts():
+  h, sx, sy, sz = 0.18, 0.3, 0.2, 0.2
+  xml = f"""<mujoco><worldbody><geom type="plane" size="5 5 .1"/>
+    <body pos="0 0 {h}"><freejoint/><geom type="box" size="{sx} {sy} {sz}"/></body></worldbody></mujoco>"""
+  dist, pos, frame, geom = _contacts(xml)
+  assert len(dist) == 4
+  np.testing.assert_allclose(dist, h - sz, atol=1e-12)
+  corners = sorted((round(p[0], 9), round(p[1], 9)) for p in pos)
+  assert corners == sorted((x, y) for x in (-sx, sx) for y in (-sy, sy))
+  np.testing.assert_allclose(pos[:, 2], (h - sz) - 0.5 * (h - sz), atol=1e-12)
+
+
+def test_contact_rows_are_contact_point_velocities():
+  """Pyramidal rows of a sliding, spinning sphere on a plane: the two rows of
+  each friction direction average to the normal row, J_n qdot is the
+  approach speed d(dist)/dt, and the half-difference over mu is the contact
+  point's velocity along that tangent of the frame."""
+  r, z = 0.1, 0.095
+  xml = f"""<mujoco><option gravity="0 0 0"/><worldbody><geom type="plane" size="5 5 .1"/>
+    <body pos="0 0 {z}"><freejoint/><geom type="sphere" size="{r}" friction="0.7 0.01 0.001"/></body>
+    </worldbody></mujoco>"""
+  mjm, m, d, hm, hd = helpers.load(xml)
+  v = np.array([0.4, -0.3, 0.2, 1.1, -0.6, 0.9])
+  hd.arrays["qvel"][0] = v
+  oracle.run("fwd_position", hm, hd)
+  assert int(hd["ncon"][0]) == 1 and int(hd["nefc"][0]) == 4
+  J = hd["efc_J"][:4, :mjm.nv]
+  frame = hd["contact_frame"][0].reshape(3, 3)
+  # the element-wise max of the two geoms' friction: the plane's default 1
+  np.testing.assert_allclose(hd["contact_friction"][0][:2], [1.0, 1.0])
+  mu = 1.0
+  p = hd["contact_pos"][0]
+  c = hd["xpos"][0][1]
+  R = hd["xmat"][0][1].reshape(3, 3)
+  # free joint: qvel = (world linear velocity of the body origin, local angular velocity)
+  w_world = R @ v[3:]
+  vp = v[:3] + np.cross(w_world, p - c)
+  Jn = 0.5 * (J[0] + J[1])
+  np.testing.assert_allclose(0.5 * (J[2] + J[3]), Jn, atol=1e-12)
+  np.testing.assert_allclose(Jn @ v, frame[0] @ vp, atol=1e-12)
+  np.testing.assert_allclose(Jn @ v, v[2], atol=1e-12)  # d(dist)/dt for a sphere on a plane
+  for k, rows in enumerate(((0, 1), (2, 3))):
+    jt = (J[rows[0]] - J[rows[1]]) / (2 * mu)
+    np.testing.assert_allclose(jt @ v, frame[1 + k] @ vp, atol=1e-12)
