@@ -423,10 +423,12 @@ int mjh_forward(const mjhModel* m, const mjhData* d, mjhStream stream);
 /* forward.py:682-695 step: one launch per stage (the plain per-kernel step).
  * Does not advance d.time, which the reference keeps on the host. */
 int mjh_step(const mjhModel* m, const mjhData* d, mjhStream stream);
-/* The same step as mjh_step, computed by one fused launch per step: every
- * stage of a world runs in one workgroup with the world's working set staged
- * in LDS.  No counterpart in the reference; results equal mjh_step's up to
- * floating-point summation order. */
+/* The same step as mjh_step with the stages fused: for nv <= 32 two
+ * launches (every forward stage; solver and integrator) on 32-lane teams,
+ * otherwise one launch per step; worlds whose constraint rows exceed the
+ * fused solver's capacity finish in a tail launch.  No counterpart in the
+ * reference; results equal mjh_step's up to floating-point summation
+ * order. */
 int mjh_step_fused(const mjhModel* m, const mjhData* d, mjhStream stream);
 /* forward.py:43-162 and 682-695 for nstep steps in one call */
 int mjh_rollout(const mjhModel* m, const mjhData* d, int nstep, int fused, mjhStream stream);

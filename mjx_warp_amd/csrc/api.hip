@@ -237,15 +237,15 @@ int lds_step(const mjhModel& m) {
   return a > b ? a : b;
 }
 
-// MJH_SOLVER_PATH=reg runs worlds of models with nv <= 32 on 32-lane teams
-// (two per wavefront) with the register solver; by default every model
-// uses 64-lane teams and the LDS solver.
+// Worlds of models with nv <= 32 run on 32-lane teams (two per wavefront)
+// with the register solver; larger models, or every model when
+// MJH_SOLVER_PATH=lds, on 64-lane teams with the LDS solver.
 bool reg_path(const mjhModel& m) {
-  static const bool use_reg = [] {
+  static const bool force_lds = [] {
     const char* e = std::getenv("MJH_SOLVER_PATH");
-    return e && std::strcmp(e, "reg") == 0;
+    return e && std::strcmp(e, "lds") == 0;
   }();
-  return m.nv <= kRegNvMax && use_reg;
+  return m.nv <= kRegNvMax && !force_lds;
 }
 // per-world LDS of the 32-lane kernels, a multiple of 4 floats so every
 // world's broadcast buffers are float4-aligned

@@ -235,9 +235,13 @@ struct Team {
 // accesses of all lanes.
 MJH_DEV void sync() { __syncthreads(); }
 
-// Sum of v over the TEAM lanes of a team; every lane gets the result.
+// Sum of v over the TEAM lanes of a team; every lane gets the same bits.
+// v is materialised (rounded) before the butterfly so that a product passed
+// in cannot be contracted into the first add differently in the two lanes
+// of a pair.
 template <int TEAM>
 MJH_DEV float team_sum(float v) {
+  asm volatile("" : "+v"(v));
 #pragma unroll
   for (int o = TEAM / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, TEAM);
   return v;

@@ -50,10 +50,8 @@ def _close(name, a, b, tol):
   assert err <= tol * scale, f"{name}: max err {err:.3e} > {tol:.1e} * {scale:.3e}"
 
 
-def make_case(mjm, nworld, seed, nconmax_w, njmax_w, key=None, qpos_noise=0.05, qvel_noise=0.5, ctrl=True,
-              f64=False):
-  """Device model/data and float32 oracle copies of one seeded batch (plus
-  float64 oracle copies when `f64`)."""
+def _inputs(mjm, nworld, seed, key=None, qpos_noise=0.05, qvel_noise=0.5, ctrl=True):
+  """Seeded batch: hinge / slide joints perturbed around the keyframe."""
   rng = np.random.default_rng(seed)
   qpos = np.tile(mjm.key_qpos[key] if key is not None else mjm.qpos0, (nworld, 1)).astype(np.float64)
   for j in range(mjm.njnt):
@@ -62,6 +60,12 @@ def make_case(mjm, nworld, seed, nconmax_w, njmax_w, key=None, qpos_noise=0.05, 
       qpos[:, qa] += rng.normal(0, qpos_noise, nworld)
   qvel = rng.normal(0, qvel_noise, (nworld, mjm.nv))
   u = rng.uniform(-1, 1, (nworld, mjm.nu)) if ctrl else np.zeros((nworld, mjm.nu))
+  return qpos, qvel, u
+
+
+def make_case(mjm, nworld, seed, nconmax_w, njmax_w, key=None, qpos_noise=0.05, qvel_noise=0.5, ctrl=True):
+  """Device model/data and float32 oracle copies of one seeded batch."""
+  qpos, qvel, u = _inputs(mjm, nworld, seed, key, qpos_noise, qvel_noise, ctrl)
   out = []
   for dev in ("cuda", "cpu"):
     m = mjw.put_model(mjm, device=dev)
@@ -73,9 +77,57 @@ def make_case(mjm, nworld, seed, nconmax_w, njmax_w, key=None, qpos_noise=0.05, 
   m, d, mc, dc = out
   hm = helpers.host_model(mc, np.float32)
   hd = helpers.host_data(mc, dc, np.float32)
-  if f64:
-    return m, d, hm, hd, helpers.host_model(mc, np.float64), helpers.host_data(mc, dc, np.float64)
   return m, d, hm, hd
+
+
+def oracle_envelope(mjm, nworld, seed, nconmax_w, njmax_w, stage, fields, key=None, qpos_noise=0.05,
+                    qvel_noise=0.5, ctrl=True, opt=None, k=8):
+  """The oracle's own rounding envelope of one seeded batch: `stage` run in
+  float64 and, k times, in float32 with the hinge / slide coordinates moved
+  by 1e-6 and qvel by a relative 1e-6 -- perturbations at the level of the
+  rounding differences between two FP32 implementations of the same step.
+  Returns {field: [nvariant, ...]}.  Truncated iterative line searches and
+  flat cost valleys make some worlds' results depend on rounding (the
+  oracle itself takes either branch of a line-search bracket decision under
+  these perturbations); the HIP result is held to this envelope there."""
+  qpos, qvel, u = _inputs(mjm, nworld, seed, key, qpos_noise, qvel_noise, ctrl)
+  rng = np.random.default_rng(12345)
+  hinge = [mjm.jnt_qposadr[j] for j in range(mjm.njnt) if mjm.jnt_type[j] in (2, 3)]
+  out = {f: [] for f in fields}
+  for i in range(k + 1):
+    real = np.float64 if i == 0 else np.float32
+    q, v = qpos.copy(), qvel
+    if i > 0:
+      q[:, hinge] += 1e-6 * rng.normal(size=(nworld, len(hinge)))
+      v = qvel * (1.0 + 1e-6 * rng.normal(size=qvel.shape))
+    mc = mjw.put_model(mjm, device="cpu")
+    dc = mjw.make_data(mjm, nworld=nworld, nconmax=nconmax_w * nworld, njmax=njmax_w * nworld, device="cpu")
+    helpers.set_field(dc, "qpos", q)
+    helpers.set_field(dc, "qvel", v)
+    helpers.set_field(dc, "ctrl", u)
+    hm = helpers.host_model(mc, real)
+    hm.opt.update(opt or {})
+    hd = helpers.host_data(mc, dc, real)
+    oracle.run(stage, hm, hd)
+    for f in fields:
+      out[f].append(np.asarray(hd[f], np.float64).copy())
+  return {f: np.stack(v) for f, v in out.items()}
+
+
+def _close_or_enveloped(name, got, ref, env, tol, min_tight=0.9):
+  """Per world: |got - ref| within tol * (1 + max|ref|), or within twice the
+  oracle's rounding envelope `env` ([nvariant, nworld, ...]) of ref.  At
+  least `min_tight` of the worlds must meet the plain tolerance."""
+  got = np.asarray(got, np.float64).reshape(ref.shape[0], -1)
+  ref = np.asarray(ref, np.float64).reshape(ref.shape[0], -1)
+  env = np.asarray(env, np.float64).reshape(env.shape[0], ref.shape[0], -1)
+  scale = 1.0 + np.max(np.abs(ref))
+  err = np.abs(got - ref).max(axis=1)
+  spread = np.abs(env - ref[None]).max(axis=(0, 2))
+  tight = err <= tol * scale
+  assert np.mean(tight) >= min_tight, f"{name}: {np.sum(~tight)} worlds over {tol:.1e} * {scale:.3e}"
+  bad = ~tight & (err > 2.0 * spread)
+  assert not np.any(bad), f"{name}: worlds {np.where(bad)[0].tolist()} err {err[bad]} spread {spread[bad]}"
 
 
 def compare_forward(m, d, hd, tol_smooth=2e-4, tol_solver=2e-3):
@@ -114,31 +166,28 @@ def test_forward_humanoid():
   compare_forward(m, d, hd)
 
 
-def _cost_parity(d, hd32, hd64, factor=1.025, cond=1e-3, max_ill=0.25, loose=1.5):
+def _cost_parity(d, c32, env, factor=1.025, min_tight=0.8):
   """Final-cost parity of a truncated solve.
 
   A solve cut off after a few iterations returns an iterate, not the
-  minimiser, and that iterate can depend on rounding: the oracle's own float32
-  and float64 runs disagree by up to 60% in cost on some worlds (Newton with
-  the iterative line search at 2 iterations: its first trial point sits where
-  the slope is zero up to rounding, so the bracket decisions flip with the
-  summation order -- the reference's sums are unordered atomics, so it is no
-  different there).  Worlds where the two oracle runs agree to `cond` are
-  well-conditioned: on them the HIP cost is within the reference's factor
-  (solver_test.py:127) of the oracle's, both ways.  They must be at least
-  1 - max_ill of all worlds.  On the others the HIP cost lies within `loose`
-  of the spread of the two oracle runs."""
+  minimiser, and on some worlds that iterate depends on rounding: the first
+  trial point of the iterative line search sits where the slope is zero up
+  to rounding, so its bracket decisions flip with the summation order (the
+  reference's sums are unordered atomics, so it is no different there).
+  Every world's HIP cost lies within the reference's factor (solver_test.py:127)
+  of the range spanned by the float32 oracle and its rounding envelope
+  (float64 and perturbed float32 runs); at least `min_tight` of the worlds
+  match the float32 oracle to 1e-3."""
   gc = helpers.data_numpy(d, "efc_cost").astype(np.float64)
-  c32 = hd32["efc_cost"].astype(np.float64)
-  c64 = hd64["efc_cost"].astype(np.float64)
-  slack = 1e-3 * (1.0 + np.abs(c64))
-  well = np.abs(c32 - c64) <= cond * np.abs(c64) + slack
-  assert np.mean(~well) <= max_ill, np.stack([c32, c64])
-  assert np.all((gc <= factor * c32 + slack)[well]), np.stack([gc, c32])
-  assert np.all((c32 <= factor * gc + slack)[well]), np.stack([gc, c32])
-  lo = np.minimum(c32, c64) / loose - slack
-  hi = np.maximum(c32, c64) * loose + slack
-  assert np.all((gc >= lo) & (gc <= hi)), np.stack([gc, c32, c64])
+  c32 = np.asarray(c32, np.float64)
+  allc = np.concatenate([c32[None], env], axis=0)
+  slack = 1e-3 * (1.0 + np.abs(c32))
+  lo = allc.min(axis=0) / factor - slack
+  hi = allc.max(axis=0) * factor + slack
+  out = (gc < lo) | (gc > hi)
+  assert not np.any(out), (np.where(out)[0].tolist(), gc[out], allc[:, out])
+  tight = np.abs(gc - c32) <= 1e-3 * np.abs(c32) + slack
+  assert np.mean(tight) >= min_tight, np.stack([gc, c32])
 
 
 # the reference's solver test configurations (solver_test.py:70-79) and the
@@ -157,17 +206,17 @@ TRUNCATED = [
 def test_forward_humanoid_truncated_solver(solver, iterations, ls_iterations, ls_parallel):
   mjm = models.humanoid(iterations=iterations, ls_iterations=ls_iterations)
   mjm.opt.solver = solver
-  m, d, hm, hd, hm64, hd64 = make_case(mjm, 64, seed=1, nconmax_w=64, njmax_w=277, key=0, f64=True)
+  m, d, hm, hd = make_case(mjm, 64, seed=1, nconmax_w=64, njmax_w=277, key=0)
   m.opt.ls_parallel = ls_parallel
-  for h in (hm, hm64):
-    h.opt["ls_parallel"] = int(ls_parallel)
+  hm.opt["ls_parallel"] = int(ls_parallel)
   mjw.forward(m, d)
   torch.cuda.synchronize()
   oracle.run("forward", hm, hd)
-  oracle.run("forward", hm64, hd64)
+  env = oracle_envelope(mjm, 64, 1, 64, 277, "forward", ["efc_cost"], key=0,
+                        opt={"ls_parallel": int(ls_parallel)})["efc_cost"]
   for f in SMOOTH_FIELDS:
     _close(f, helpers.data_numpy(d, f), hd[f], 2e-4)
-  _cost_parity(d, hd, hd64)
+  _cost_parity(d, hd["efc_cost"], env)
 
 
 def test_forward_chain():
@@ -182,19 +231,24 @@ def test_forward_chain():
 @pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("model", ["humanoid", "chain"])
 def test_step(model, fused):
+  """One plain / fused step.  The chain's CG solve stops on a flat cost
+  valley, where the oracle itself moves qvel by up to ~7e-3 under 1e-6
+  input perturbations; worlds over the plain tolerance are held to twice
+  that envelope."""
   if model == "humanoid":
     mjm = models.humanoid(iterations=10, ls_iterations=20)
-    m, d, hm, hd = make_case(mjm, 16, seed=2, nconmax_w=64, njmax_w=277, key=0)
+    args = dict(nworld=16, seed=2, nconmax_w=64, njmax_w=277, key=0)
   else:
     mjm = models.chain(3)
-    m, d, hm, hd = make_case(mjm, 64, seed=0, nconmax_w=8, njmax_w=35, qpos_noise=0.5, qvel_noise=0.1,
-                             ctrl=False)
+    args = dict(nworld=64, seed=0, nconmax_w=8, njmax_w=35, qpos_noise=0.5, qvel_noise=0.1, ctrl=False)
+  m, d, hm, hd = make_case(mjm, **args)
   (mjw.step_fused if fused else mjw.step)(m, d)
   torch.cuda.synchronize()
   oracle.run("step", hm, hd)
-  _close("qpos", helpers.data_numpy(d, "qpos"), hd["qpos"], 1e-4)
-  _close("qvel", helpers.data_numpy(d, "qvel"), hd["qvel"], 2e-3)
-  _close("qacc", helpers.data_numpy(d, "qacc"), hd["qacc"], 2e-3)
+  env = oracle_envelope(mjm, stage="step", fields=["qpos", "qvel", "qacc"], **args)
+  _close_or_enveloped("qpos", helpers.data_numpy(d, "qpos"), hd["qpos"], env["qpos"], 1e-4)
+  _close_or_enveloped("qvel", helpers.data_numpy(d, "qvel"), hd["qvel"], env["qvel"], 2e-3)
+  _close_or_enveloped("qacc", helpers.data_numpy(d, "qacc"), hd["qacc"], env["qacc"], 2e-3)
   assert abs(d.time - mjm.opt.timestep) < 1e-12
 
 
