@@ -11,8 +11,11 @@ steps).
 One step = the whole of mujoco_warp.step for all B worlds (forward
 kinematics, inertia, collision, constraint rows, solver, integration) plus
 the ctrl resample.  `value` is B * steps / wall time of the timed loop with
-all state resident in HBM; the fused step (one launch per step) is the
-headline, the plain per-stage step is reported beside it.
+all state resident in HBM; the fused step (a forward launch and a
+solve+integrate launch, plus a tail for worlds over the solver's row
+capacity) is the headline, the plain per-stage step is reported beside it.
+`roofline.traffic` is the fused step's HBM traffic measured with rocprofv3
+for the default configuration (profiles/step_humanoid_b8192_newton10.md).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -47,6 +50,25 @@ def algorithmic_bytes(mjm, ncon_w: float, nefc_w: float) -> float:
   W = (mjm.nq + 23 * mjm.nv + 57 * mjm.nbody + 6 * mjm.njnt + 12 * mjm.ngeom + 12 * mjm.nsite + 3 * mjm.nu
        + mjm.nu * mjm.nv + 2 * mjm.na + 2 * mjm.nv * mjm.nv)
   return 4.0 * (R + W + 30 * ncon_w + (mjm.nv + 6) * nefc_w)
+
+
+PROFILE = os.path.join(ROOT, "profiles", "step_humanoid_b8192_newton10.md")
+
+
+def measured_traffic(args):
+  """HBM bytes of one fused step measured with rocprofv3 FETCH_SIZE /
+  WRITE_SIZE (scaled by a plain copy's counters) for the default workload,
+  as recorded in the profile summary; None for other configurations."""
+  if (args.batch, args.iterations, args.ls_iterations) != (8192, 10, 20):
+    return None
+  try:
+    with open(PROFILE) as f:
+      for line in f:
+        if line.startswith("fused step traffic per launch"):
+          return float(line.rsplit(":", 1)[1])
+  except OSError:
+    pass
+  return None
 
 
 def setup(args, device="cuda"):
@@ -221,9 +243,10 @@ def main():
                "ls_iterations": args.ls_iterations, "integrator": "implicitfast", "timestep": 0.005,
                "gear": 100, "nconmax_per_world": 64, "njmax_per_world": 277},
     "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                 "kernel": "step_fused (counter reset + fused step launch)",
-                 "algorithmic_bytes_per_world_step": bytes_ws, "kernel_ms": fused["kernel_ms"]},
+                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
+                 "kernel": "fused step: counter reset, forward launch, solve+integrate launch, tail",
+                 "algorithmic_bytes_per_world_step": bytes_ws, "worlds_per_launch": args.batch,
+                 "kernel_ms": fused["kernel_ms"]},
     "cpu_baseline": base,
     "mean_ncon_per_world": fused["ncon_w"],
     "mean_nefc_per_world": fused["nefc_w"],
