@@ -128,17 +128,18 @@ struct StFusedStep {
   }
 };
 
-// The fused step in two launches, for 32-lane teams: the forward stages up
-// to qacc_smooth, then the solver and the integrator.  One kernel holding
-// both would need the register and SGPR budget of the largest stage for
-// the whole step.
+// The fused step in two launches, for 32-lane teams: the forward stages
+// that walk the body tree, collide and build constraint rows (up to
+// qfrc_smooth), then everything that factors or solves with the mass
+// matrix: factor_m, qacc_smooth, the solver and the integrator.  The second
+// group keeps a world's matrix rows in VGPRs; one kernel holding both
+// groups would impose that register budget on the whole step.
 struct StFusedForward {
   template <int T>
   MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune&) const {
     kinematics(m, d, t);
     com_pos(m, d, t);
     crb(m, d, t);
-    factor_m(m, d, l, t);
     sync();
     collision(m, d, l, t);
     make_constraint(m, d, t);
@@ -146,12 +147,15 @@ struct StFusedForward {
     transmission(m, d, t);
     fwd_velocity(m, d, t);
     fwd_actuation(m, d, t);
-    fwd_acceleration(m, d, l, t);
+    fwd_acceleration_force(m, d, t);
   }
 };
 struct StFusedSolveIntegrate {
   template <int T>
   MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
+    factor_m(m, d, l, t);
+    sync();
+    fwd_acceleration_solve(m, d, l, t);
     if (d.njmax == 0) {
       if (t.valid)
         for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] = d.qacc_smooth[t.world * m.nv + i];
@@ -253,11 +257,6 @@ int lds_solve_reg(const mjhModel& m) {
   const int a = lds_reg_floats(tune_for(m).rj), b = lds_linalg(m);
   return ((a > b ? a : b) + 3) & ~3;
 }
-int lds_step_reg(const mjhModel& m) {
-  const int a = lds_solve_reg(m);
-  const int b = m.ngeom > 100 ? lds_sap(m) : 0;
-  return ((a > b ? a : b) + 3) & ~3;
-}
 
 int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
@@ -265,8 +264,8 @@ int check_launch(const char* what) {
   return MJH_OK;
 }
 
-template <class S, int TEAM = kTeam>
-int launch(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, const char* name) {
+template <class S, int TEAM>
+int launch_team(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, const char* name) {
   if (d.nworld <= 0) return MJH_OK;
   constexpr int wpb = 64 / TEAM;
   const int grid = (d.nworld + wpb - 1) / wpb;
@@ -276,6 +275,14 @@ int launch(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, const c
                                       std::to_string(m.nv) + ")");
   hipLaunchKernelGGL((k_stage<TEAM, S>), dim3(grid), dim3(64), bytes, s, m, d, ldsw, tune_for(m));
   return check_launch(name);
+}
+
+// A stage on the model's team size: 32-lane teams (two worlds per
+// wavefront) when the register path applies, else 64-lane teams.
+template <class S>
+int launch(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, const char* name) {
+  if (reg_path(m)) return launch_team<S, kRegTeam>(m, d, s, (ldsw + 3) & ~3, name);
+  return launch_team<S, kTeam>(m, d, s, ldsw, name);
 }
 
 int reset(const mjhData& d, hipStream_t s, int which) {
@@ -295,9 +302,9 @@ int do_collision(const mjhModel& m, const mjhData& d, hipStream_t s) {
 }
 
 int do_solve(const mjhModel& m, const mjhData& d, hipStream_t s) {
-  if (reg_path(m)) TRY((launch<StSolveTeam, kRegTeam>(m, d, s, lds_solve_reg(m), "solve")));
-  else TRY(launch<StSolveTeam>(m, d, s, lds_solve(m), "solve"));
-  return launch<StTail<false>>(m, d, s, lds_linalg(m), "solve_tail");
+  if (reg_path(m)) TRY((launch_team<StSolveTeam, kRegTeam>(m, d, s, lds_solve_reg(m), "solve")));
+  else TRY((launch_team<StSolveTeam, kTeam>(m, d, s, lds_solve(m), "solve")));
+  return launch_team<StTail<false>, kTeam>(m, d, s, lds_linalg(m), "solve_tail");
 }
 
 int do_fwd_position(const mjhModel& m, const mjhData& d, hipStream_t s) {
@@ -334,12 +341,12 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
     return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
   TRY(reset(d, s, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3));
   if (reg_path(m)) {
-    TRY((launch<StFusedForward, kRegTeam>(m, d, s, lds_step_reg(m), "step_fused_forward")));
-    TRY((launch<StFusedSolveIntegrate, kRegTeam>(m, d, s, lds_solve_reg(m), "step_fused_solve")));
+    TRY((launch_team<StFusedForward, kRegTeam>(m, d, s, m.ngeom > 100 ? lds_sap(m) : 0, "step_fused_forward")));
+    TRY((launch_team<StFusedSolveIntegrate, kRegTeam>(m, d, s, lds_solve_reg(m), "step_fused_solve")));
   } else {
-    TRY(launch<StFusedStep>(m, d, s, lds_step(m), "step_fused"));
+    TRY((launch_team<StFusedStep, kTeam>(m, d, s, lds_step(m), "step_fused")));
   }
-  return launch<StTail<true>>(m, d, s, lds_linalg(m), "step_tail");
+  return launch_team<StTail<true>, kTeam>(m, d, s, lds_linalg(m), "step_tail");
 }
 
 }  // namespace

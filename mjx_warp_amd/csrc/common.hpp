@@ -265,4 +265,16 @@ MJH_DEV int team_exscan_int(int v, int lane, int* total) {
   return incl - v;
 }
 
+// Iterates the dense per-tree blocks (qLD_tile / qLD_tileadr / qLD_tilesize,
+// io.py:150-162) and calls f(dofid, size) for each.
+template <class F>
+MJH_DEV void for_each_tree_block(const mjhModel& m, F f) {
+  for (int g = 0; g < m.nqLD_group; g++) {
+    const int beg = m.qLD_tileadr[g];
+    const int end = (g == m.nqLD_group - 1) ? m.nqLD_tile : m.qLD_tileadr[g + 1];
+    const int size = m.qLD_tilesize[g];
+    for (int k = beg; k < end; k++) f(m.qLD_tile[k], size);
+  }
+}
+
 }  // namespace mjh

@@ -134,9 +134,10 @@ MJH_DEV void xfrc_accumulate(const mjhModel& m, const mjhData& d, float* qfrc_al
   }
 }
 
-// forward.py:644-661 fwd_acceleration; lds: nv*nv + nv floats
+// forward.py:644-661 fwd_acceleration, in two parts: the smooth force
+// qfrc_smooth = passive - bias + actuator + applied + J' xfrc_applied ...
 template <int TEAM>
-MJH_DEV void fwd_acceleration(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
+MJH_DEV void fwd_acceleration_force(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
   const int w = t.world, nv = m.nv;
   float* qs = WPTR(d.qfrc_smooth, w, nv);
   if (t.valid) {
@@ -149,8 +150,20 @@ MJH_DEV void fwd_acceleration(const mjhModel& m, const mjhData& d, float* lds, c
   sync();
   xfrc_accumulate(m, d, d.qfrc_smooth, t);
   sync();
-  if (t.valid) solve_blocks(m, WPTR(d.qLD, w, nv * nv), WPTR(d.qacc_smooth, w, nv), qs, lds, t);
+}
+
+// ... and qacc_smooth = inv(M) qfrc_smooth with the factor qLD; lds: nv*nv + nv floats
+template <int TEAM>
+MJH_DEV void fwd_acceleration_solve(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
+  const int w = t.world, nv = m.nv;
+  if (t.valid) solve_blocks(m, WPTR(d.qLD, w, nv * nv), WPTR(d.qacc_smooth, w, nv), WPTR(d.qfrc_smooth, w, nv), lds, t);
   sync();
+}
+
+template <int TEAM>
+MJH_DEV void fwd_acceleration(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
+  fwd_acceleration_force(m, d, t);
+  fwd_acceleration_solve(m, d, lds, t);
 }
 
 // support.py:35-93 mul_m (dense): res = M vec over the tree blocks; worlds
@@ -224,6 +237,29 @@ MJH_DEV void euler(const mjhModel& m, const mjhData& d, float* lds, const Team<T
     sync();
     return;
   }
+  if constexpr (TEAM == kRegTeam) {
+    // (M + dt diag(damping)) qacc_int = qfrc_smooth + qfrc_constraint, the
+    // matrix row and its factor in registers
+    if (t.valid) {
+      const int lane = t.lane;
+      const bool dof = lane < nv;
+      int beg, end;
+      tree_block(m, lane, &beg, &end);
+      const float damp = dof ? m.dof_damping[lane] * m.opt.timestep : 0.0f;
+      const float b = dof ? d.qfrc_smooth[w * nv + lane] + d.qfrc_constraint[w * nv + lane] : 0.0f;
+      const float x = with_nvc(nv, [&](auto nvc) {
+        RegChol<decltype(nvc)::value> c{lane, lds};
+        c.load_sym(WPTR(d.qM, w, nv * nv), nv, beg, end, damp);
+        c.factor();
+        return c.solve(b);
+      });
+      if (dof) d.qacc_integration[w * nv + lane] = x;
+    }
+    sync();
+    advance(m, d, d.qacc_integration, t);
+    sync();
+    return;
+  }
   if (t.valid) {
     const float* qM = WPTR(d.qM, w, nv * nv);
     const float* qs = WPTR(d.qfrc_smooth, w, nv);
@@ -278,6 +314,48 @@ MJH_DEV void implicit(const mjhModel& m, const mjhData& d, float* lds, const Tea
     }
   }
   sync();
+  if constexpr (TEAM == kRegTeam) {
+    // qM_integration = qM - dt (moment' diag(vel) moment - diag(damping)),
+    // a row per lane in registers, written out, factored and solved in place
+    if (t.valid) {
+      const int lane = t.lane;
+      const bool dof = lane < nv;
+      int beg, end;
+      tree_block(m, lane, &beg, &end);
+      const float damp = (dof && passive_on) ? m.dof_damping[lane] * dt : 0.0f;
+      const float b = dof ? d.qfrc_smooth[w * nv + lane] + d.qfrc_constraint[w * nv + lane] : 0.0f;
+      if (dof) d.qfrc_integration[w * nv + lane] = b;
+      const float* moment = WPTR(d.actuator_moment, w, nu * nv);
+      float* qMi = WPTR(d.qM_integration, w, nv * nv);
+      const float x = with_nvc(nv, [&](auto nvc) {
+        constexpr int NVC = decltype(nvc)::value;
+        RegChol<NVC> c{lane, lds};
+        c.load_sym(WPTR(d.qM, w, nv * nv), nv, beg, end, damp);
+        if (act_on) {
+          for (int a = 0; a < nu; a++) {
+            const float* mr = moment + a * nv;
+            const float ca = dof ? dt * mr[lane] * avi[a] : 0.0f;
+            if (ca == 0.0f) continue;
+#pragma unroll
+            for (int j = 0; j < NVC; j++)
+              if (j < nv && j >= beg && j < end) c.h[j] -= ca * mr[j];
+          }
+        }
+        if (dof) {
+#pragma unroll
+          for (int j = 0; j < NVC; j++)
+            if (j >= beg && j < end) qMi[lane * nv + j] = c.h[j];
+        }
+        c.factor();
+        return c.solve(b);
+      });
+      if (dof) d.qacc_integration[w * nv + lane] = x;
+    }
+    sync();
+    advance(m, d, d.qacc_integration, t);
+    sync();
+    return;
+  }
   if (t.valid) {
     const float* qM = WPTR(d.qM, w, nv * nv);
     const float* moment = WPTR(d.actuator_moment, w, nu * nv);

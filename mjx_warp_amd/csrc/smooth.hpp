@@ -9,6 +9,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "linalg_reg.hpp"
 
 namespace mjh {
 
@@ -289,22 +290,14 @@ MJH_DEV void cho_solve_lds(const float* L, int n, int ld, float* x, const Team<T
   }
 }
 
-// Iterates the dense per-tree blocks (qLD_tile / qLD_tileadr / qLD_tilesize,
-// io.py:150-162) and calls f(dofid, size) for each.
-template <class F>
-MJH_DEV void for_each_tree_block(const mjhModel& m, F f) {
-  for (int g = 0; g < m.nqLD_group; g++) {
-    const int beg = m.qLD_tileadr[g];
-    const int end = (g == m.nqLD_group - 1) ? m.nqLD_tile : m.qLD_tileadr[g + 1];
-    const int size = m.qLD_tilesize[g];
-    for (int k = beg; k < end; k++) f(m.qLD_tile[k], size);
-  }
-}
-
 // Factor the tree blocks of M (global, [nv, nv]) into L (global, may alias
 // a different array) through an LDS staging buffer of nv*nv floats.
 template <int TEAM>
 MJH_DEV void factor_blocks(const mjhModel& m, const float* M, float* L, float* lds, const Team<TEAM>& t) {
+  if constexpr (TEAM == kRegTeam) {
+    reg_factor_blocks(m, M, L, lds, t);
+    return;
+  }
   const int nv = m.nv;
   for_each_tree_block(m, [&](int dof, int n) {
     for (int p = t.lane; p < n * n; p += TEAM) lds[p] = M[(dof + p / n) * nv + dof + p % n];
@@ -320,6 +313,10 @@ MJH_DEV void factor_blocks(const mjhModel& m, const float* M, float* L, float* l
 template <int TEAM>
 MJH_DEV void solve_blocks(const mjhModel& m, const float* L, float* x, const float* y, float* lds,
                           const Team<TEAM>& t) {
+  if constexpr (TEAM == kRegTeam) {
+    reg_solve_blocks(m, L, x, y, t);
+    return;
+  }
   const int nv = m.nv;
   for_each_tree_block(m, [&](int dof, int n) {
     float* Ls = lds;

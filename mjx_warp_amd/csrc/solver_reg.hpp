@@ -25,66 +25,16 @@
 // count and no guards; padded entries never change a sum.
 #pragma once
 
+#include "linalg_reg.hpp"
 #include "solver_lds.hpp"
 
 namespace mjh {
 
-constexpr int kRegTeam = 32;
-constexpr int kRegNvMax = 32;
 constexpr int kRegRows = 64;  // row capacity: two rows per lane
 
 // LDS floats per world: broadcast buffers (vector 32, column 32, rows 64)
 // and rj staged Jacobian rows of stride 33, rounded up to a float4.
 __host__ __device__ inline int lds_reg_floats(int rj) { return (128 + rj * 33 + 3) & ~3; }
-
-MJH_DEV void wave_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  asm volatile("" ::: "memory");
-}
-
-template <int CTRL>
-MJH_DEV float dpp_f(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
-}
-
-// Sum over the 32 lanes of a team (all lanes get identical bits: every
-// step adds the same two rounded values in both lanes of a pair).  The
-// input is materialised first, so a product passed in is rounded before the
-// butterfly: contracting it into the first add as fma(a, b, partner) would
-// give the two lanes of a pair different bits, and team-wide decisions
-// taken on the sum (line-search bracket, step length) would diverge.
-MJH_DEV float sum32(float v) {
-  asm volatile("" : "+v"(v));
-  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
-  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
-  v += dpp_f<0x141>(v);  // row_half_mirror: 8-lane halves of a row
-  v += dpp_f<0x140>(v);  // row_mirror: the two 8-lane halves of a row
-  // bitmask swizzle inside 32-lane groups: and 0x1f, or 0, xor 0x10
-  v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
-  return v;
-}
-
-// p, opaque to the optimiser: address arithmetic on it is redone at each use
-// instead of being hoisted out of the solver loop into live VGPRs.
-typedef __attribute__((address_space(1))) const float gfloat;
-MJH_DEV gfloat* opaque(const float* p) {
-  gfloat* q = (gfloat*)p;
-  asm volatile("" : "+v"(q));
-  return q;
-}
-MJH_DEV int opaque(int i) {
-  asm volatile("" : "+v"(i));
-  return i;
-}
-
-// Value of v in lane k of this lane's team (k wave-uniform).
-MJH_DEV float bcast32(float v, int k) {
-  const float lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
-  const float hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k + 32));
-  return (threadIdx.x & 32) ? hi : lo;
-}
 
 template <int NVC>
 struct RegSolver {
@@ -93,18 +43,17 @@ struct RegSolver {
   const mjhData& d;
   int w, nv, radr, nr, lane, rj;
   float* vbuf;  // [32] dof-vector broadcast
-  float* cbuf;  // [32] Cholesky column broadcast
   float* fbuf;  // [64] row-scalar broadcast
   float* Jl;    // [rj][33] staged Jacobian rows
   const float* M;  // qM of the world (global, read-only during the solve)
   // dof lane state
-  float h[NVC];  // Hessian row / Cholesky factor row (lower part)
-  float rdiag;      // 1 / L[lane][lane]
+  RegChol<NVC> ch;  // Hessian row and its Cholesky factor (CG: factor of M)
   float qacc, Ma, grad, Mgrad, search, mv, qs, qas, qc;
   // row lane state: rows lane and lane + 32
   float Jaref[2], jv[2], D[2], force[2], q0[2], q1[2], q2[2];
 
-  int blk;  // CG: first dof of the lane's tree block
+  int blk;   // CG: first dof of the lane's tree block
+  bool two;  // rows past 32 exist: the second row set is live (team-uniform)
 
   MJH_DEV int row(int s) const { return lane + 32 * s; }
   // global Jacobian row r (rows past the LDS capacity)
@@ -149,67 +98,12 @@ struct RegSolver {
     return s;
   }
 
-  // Cholesky H = L L' of the lane-owned rows h (solver.py:250-276; the
-  // reference factors the full nv x nv tile).  Lane i keeps L[i][0..i] in
-  // h[0..i] and 1 / L[i][i] in rdiag.
-  MJH_DEV void factor() {
-#pragma unroll
-    for (int k = 0; k < NVC; k++) {
-      wave_fence();
-      cbuf[lane] = h[k];
-      wave_fence();
-      const float dkk = cbuf[k];
-      const float lkk = __builtin_amdgcn_sqrtf(dkk);
-      const float inv = __builtin_amdgcn_rcpf(lkk);
-      // L[i][k] L[j][k] = H[i][k] H[j][k] / dkk
-      const float g = lane > k ? h[k] * (inv * inv) : 0.0f;
-#pragma unroll
-      for (int j = k + 1; j < NVC; j++) h[j] = fmaf(-g, cbuf[j], h[j]);
-      if (lane == k) rdiag = inv;
-      h[k] = lane > k ? h[k] * inv : (lane == k ? lkk : h[k]);
-    }
-  }
-
-  // CG: the factor of M (qLD) by kinematic-tree blocks into h / rdiag;
-  // entries outside the lane's block are zero, padded dofs identity rows.
-  // Reloaded at each use (it is read-only), so h is dead between solves.
-  MJH_DEV void load_qld() {
-    const int ln = opaque(lane);
-    const bool dof = ln < nv;
-    gfloat* L = opaque(d.qLD + (size_t)w * nv * nv) + (dof ? ln * nv : 0);
-#pragma unroll
-    for (int j = 0; j < NVC; j++) {
-      const float v = L[j < nv ? j : 0];
-      h[j] = (dof && j >= blk && j <= ln) ? v : (j == ln ? 1.0f : 0.0f);
-    }
-    rdiag = dof ? 1.0f / L[ln] : 1.0f;
-  }
-
-  // x = inv(L L') b with L from factor(); b, x one entry per lane
-  MJH_DEV float cho_solve(float b) const {
-    // forward: y = inv(L) b, column by column
-    float y = b;
-#pragma unroll
-    for (int k = 0; k < NVC; k++) {
-      const float yk = bcast32(y * rdiag, k);
-      y = lane > k ? fmaf(-h[k], yk, y) : y;
-    }
-    y *= rdiag;
-    // backward: x = inv(L') y, x[k] = (y[k] - sum_{j>k} L[j][k] x[j]) / L[k][k]
-    float x = 0.0f;
-#pragma unroll
-    for (int k = NVC - 1; k >= 0; k--) {
-      const float s = sum32(lane > k ? h[k] * x : 0.0f);
-      if (lane == k) x = (y - s) * rdiag;
-    }
-    return x;
-  }
-
   // solver.py:75-173: active set, forces, cost; returns cost and gauss
   MJH_DEV void update_constraint(float* cost, float* gauss) {
     float c = 0.0f;
 #pragma unroll
     for (int s = 0; s < 2; s++) {
+      if (s == 1 && !two) break;
       const float jar = Jaref[s];
       const bool a = jar < 0.0f;
       force[s] = a ? -1.0f * D[s] * jar : 0.0f;
@@ -241,7 +135,7 @@ struct RegSolver {
 #pragma unroll
       for (int j = 0; j < NVC; j++) {
         const float v = Mc[(j < nv ? j : 0) * nv];
-        h[j] = (ln < nv && j < nv) ? v : (j == ln ? 1.0f : 0.0f);
+        ch.h[j] = (ln < nv && j < nv) ? v : (j == ln ? 1.0f : 0.0f);
       }
       wave_fence();
       fbuf[lane] = Jaref[0] < 0.0f ? D[0] : 0.0f;
@@ -254,7 +148,7 @@ struct RegSolver {
         const float* Jr = Jl + r * 33;
         const float a = dr * Jr[lane];
 #pragma unroll
-        for (int j = 0; j < NVC; j++) h[j] = fmaf(a, Jr[j], h[j]);
+        for (int j = 0; j < NVC; j++) ch.h[j] = fmaf(a, Jr[j], ch.h[j]);
       }
       const int c = lane < nv ? lane : 0;
       for (int r = nl; r < nr; r++) {
@@ -263,13 +157,13 @@ struct RegSolver {
         gfloat* Jr = Jg(r);
         const float a = lane < nv ? dr * Jr[c] : 0.0f;
 #pragma unroll
-        for (int j = 0; j < NVC; j++) h[j] = fmaf(a, j < nv ? Jr[j < nv ? j : 0] : 0.0f, h[j]);
+        for (int j = 0; j < NVC; j++) ch.h[j] = fmaf(a, j < nv ? Jr[j < nv ? j : 0] : 0.0f, ch.h[j]);
       }
-      factor();
+      ch.factor();
     } else {
-      load_qld();
+      ch.load_lower(d.qLD + (size_t)w * nv * nv, nv, blk);
     }
-    Mgrad = cho_solve(grad);
+    Mgrad = ch.solve(grad);
     return gd;
   }
 
@@ -277,6 +171,7 @@ struct RegSolver {
     float a = 0.f, b = 0.f, c = 0.f;
 #pragma unroll
     for (int s = 0; s < 2; s++)
+      if (s == 0 || two)
       if (Jaref[s] + alpha * jv[s] < 0.0f) {
         const V3f p = eval_pt(q0[s], q1[s], q2[s], alpha);
         a += p.a;
@@ -292,6 +187,7 @@ struct RegSolver {
     float pa = 0.f, pb = 0.f, pc = 0.f;
 #pragma unroll
     for (int s = 0; s < 2; s++)
+      if (s == 0 || two)
       if (Jaref[s] < 0.0f) {
         pa += q0[s];
         pb += q1[s];
@@ -320,6 +216,7 @@ struct RegSolver {
       float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f, s8 = 0.f;
 #pragma unroll
       for (int s = 0; s < 2; s++) {
+        if (s == 1 && !two) break;
         const float jar = Jaref[s], v = jv[s];
         if (jar + la * v < 0.0f) {
           const V3f p = eval_pt(q0[s], q1[s], q2[s], la);
@@ -362,6 +259,7 @@ struct RegSolver {
       float s0 = 0.f, s1 = 0.f, s2 = 0.f;
 #pragma unroll
       for (int s = 0; s < 2; s++)
+        if (s == 0 || two)
         if (Jaref[s] + a * jv[s] < 0.0f) {
           s0 += q0[s];
           s1 += q1[s];
@@ -390,6 +288,7 @@ struct RegSolver {
     mv = mrow_dot();
 #pragma unroll
     for (int s = 0; s < 2; s++) {
+      if (s == 1 && !two) break;
       const float v = jrow_dot(row(s));
       const float jar = Jaref[s], dd = D[s];
       jv[s] = v;
@@ -405,7 +304,8 @@ struct RegSolver {
     qacc += alpha * search;
     Ma += alpha * mv;
 #pragma unroll
-    for (int s = 0; s < 2; s++) Jaref[s] += alpha * jv[s];
+    for (int s = 0; s < 2; s++)
+      if (s == 0 || two) Jaref[s] += alpha * jv[s];
     return alpha;
   }
 };
@@ -419,12 +319,14 @@ MJH_DEV bool solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int r
   if (nr > kRegRows) return false;
   RegSolver<NVC> s{m, d, w, nv, radr, nr, t.lane, rj};
   s.vbuf = lds;
-  s.cbuf = lds + 32;
+  s.ch.lane = t.lane;
+  s.ch.cbuf = lds + 32;
   s.fbuf = lds + 64;
   s.Jl = lds + 128;
+  s.two = nr > 32;
   const int lane = t.lane;
   const bool dof = lane < nv;
-  s.rdiag = 1.0f;
+  s.ch.rdiag = 1.0f;
   s.mv = 0.0f;
   const size_t o = (size_t)w * nv;
   // stage the world's inputs (solver.py:973 warm start)
