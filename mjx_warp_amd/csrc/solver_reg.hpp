@@ -54,6 +54,11 @@ struct RegSolver {
 
   int blk;   // CG: first dof of the lane's tree block
   bool two;  // rows past 32 exist: the second row set is live (team-uniform)
+  // the factor in ch is current: Newton's H = M + J' D(active) J depends on
+  // the iterate only through the active set, so while no row changes
+  // activity the previous factor is the one a rebuild would produce
+  bool fact_valid;
+  bool act_prev[2];
 
   MJH_DEV int row(int s) const { return lane + 32 * s; }
   // global Jacobian row r (rows past the LDS capacity)
@@ -130,6 +135,15 @@ struct RegSolver {
     grad = Ma - qs - qc;
     const float gd = sum32(grad * grad);
     if (m.opt.solver == mjhSOL_NEWTON) {
+      const bool a0 = Jaref[0] < 0.0f, a1 = two && Jaref[1] < 0.0f;
+      const float changed = sum32((a0 != act_prev[0] || a1 != act_prev[1]) ? 1.0f : 0.0f);
+      if (fact_valid && changed == 0.0f) {
+        Mgrad = ch.solve(grad);
+        return gd;
+      }
+      act_prev[0] = a0;
+      act_prev[1] = a1;
+      fact_valid = true;
       const int ln = opaque(lane);
       gfloat* Mc = opaque(M) + (ln < nv ? ln : 0);
 #pragma unroll
@@ -160,8 +174,10 @@ struct RegSolver {
         for (int j = 0; j < NVC; j++) ch.h[j] = fmaf(a, j < nv ? Jr[j < nv ? j : 0] : 0.0f, ch.h[j]);
       }
       ch.factor();
-    } else {
+    } else if (!fact_valid) {
+      // CG preconditions with the factor of M, fixed during the solve
       ch.load_lower(d.qLD + (size_t)w * nv * nv, nv, blk);
+      fact_valid = true;
     }
     Mgrad = ch.solve(grad);
     return gd;
@@ -324,6 +340,8 @@ MJH_DEV bool solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int r
   s.fbuf = lds + 64;
   s.Jl = lds + 128;
   s.two = nr > 32;
+  s.fact_valid = false;
+  s.act_prev[0] = s.act_prev[1] = false;
   const int lane = t.lane;
   const bool dof = lane < nv;
   s.ch.rdiag = 1.0f;
