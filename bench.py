@@ -95,6 +95,11 @@ def time_form(args, fused: bool):
   gen = torch.Generator(device="cuda")
   gen.manual_seed(1)
   stepfn = mjw.step_fused if fused else mjw.step
+  if args.graph:
+    # one captured step replayed per timed step (the reference benchmark's
+    # form, test_util.py:99-118); ctrl is resampled eagerly between replays
+    graph = mjw.StepGraph(m, d, nstep=1, fused=fused)
+    stepfn = lambda m_, d_: graph.replay()
   u = torch.empty_like(d.ctrl)
 
   def one_step():
@@ -196,6 +201,7 @@ def main():
   ap.add_argument("--ls_iterations", type=int, default=20)
   ap.add_argument("--no-plain", action="store_true", help="skip timing the plain per-stage step")
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+  ap.add_argument("--graph", action="store_true", help="replay a HIP graph of one captured step")
   ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                   help="only run a plain copy of MIB MiB (counter calibration) and exit")
   args = ap.parse_args()
@@ -241,7 +247,8 @@ def main():
     "config": {"workload": "humanoid 27-dof, fused step, ctrl ~ U(-1,1) resampled per step",
                "nworld": args.batch, "solver": "newton", "iterations": args.iterations,
                "ls_iterations": args.ls_iterations, "integrator": "implicitfast", "timestep": 0.005,
-               "gear": 100, "nconmax_per_world": 64, "njmax_per_world": 277},
+               "gear": 100, "nconmax_per_world": 64, "njmax_per_world": 277,
+               "launch": "hip graph replay" if args.graph else "eager"},
     "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
                  "kernel": "fused step: counter reset, forward launch, solve+integrate launch, tail",

@@ -30,6 +30,7 @@ from .api import solve as solve
 from .api import solve_m as solve_m
 from .api import step as step
 from .api import step_fused as step_fused
+from .api import StepGraph as StepGraph
 from .api import transmission as transmission
 from .api import xfrc_accumulate as xfrc_accumulate
 from .io import get_data_into as get_data_into

@@ -164,3 +164,37 @@ def rollout(m: Model, d: Data, nstep: int, fused: bool = True):
   """nstep consecutive steps issued by one library call."""
   _native.call("mjh_rollout", m, d, ctypes.c_int(int(nstep)), ctypes.c_int(int(bool(fused))))
   d.time = d.time + nstep * m.opt.timestep
+
+
+class StepGraph:
+  """`nstep` consecutive steps captured once as a HIP graph and replayed.
+
+  The counterpart of the reference benchmark's captured step
+  (test_util.py:99-118, wp.capture_begin / capture_launch): the step's
+  launches are recorded on a capture stream (the library allocates nothing
+  and records no memset nodes, so a step is capturable) and `replay()`
+  launches them on the current stream.  Replays read and write the same
+  Model / Data tensors that were captured; `d.time` is advanced on the
+  host by each replay, as `step` does.
+  """
+
+  def __init__(self, m: Model, d: Data, nstep: int = 1, fused: bool = True):
+    import torch
+
+    if m.opt.integrator not in (0, 3):
+      raise NotImplementedError(f"integrator {m.opt.integrator} not implemented.")
+    self.m, self.d, self.nstep = m, d, int(nstep)
+    name = "mjh_step_fused" if fused else "mjh_step"
+    _native.check(m, d)
+    object.__setattr__(d, "_checked", m)
+    self.graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.graph(self.graph, stream=side):
+      for _ in range(self.nstep):
+        _native.call(name, m, d)
+    torch.cuda.current_stream().wait_stream(side)
+
+  def replay(self):
+    self.graph.replay()
+    self.d.time = self.d.time + self.nstep * self.m.opt.timestep
