@@ -34,6 +34,8 @@ from .api import StepGraph as StepGraph
 from .api import transmission as transmission
 from .api import xfrc_accumulate as xfrc_accumulate
 from .io import get_data_into as get_data_into
+from .tracing import EventTracer as EventTracer
+from .tracing import event_scope as event_scope
 from .io import is_sparse as is_sparse
 from .io import make_data as make_data
 from .io import put_data as put_data
