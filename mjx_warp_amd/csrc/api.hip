@@ -52,20 +52,20 @@ struct Tune {
     }                                                                                                  \
   };
 
-STAGE(StKinematics, kinematics(m, d, t))
-STAGE(StComPos, com_pos(m, d, t))
-STAGE(StCrb, crb(m, d, t))
+STAGE(StKinematics, kinematics(m, d, l, t))
+STAGE(StComPos, com_pos(m, d, l, t))
+STAGE(StCrb, crb(m, d, l, t))
 STAGE(StFactorM, factor_m(m, d, l, t))
-STAGE(StRne, rne(m, d, t))
+STAGE(StRne, rne(m, d, l, t))
 STAGE(StTransmission, transmission(m, d, t))
-STAGE(StComVel, com_vel(m, d, t))
+STAGE(StComVel, com_vel(m, d, l, t))
 STAGE(StPassive, passive(m, d, t))
-STAGE(StNxn, nxn_broadphase(m, d, t))
+STAGE(StNxn, nxn_broadphase(m, d, l, t))
 STAGE(StSap, sap_broadphase(m, d, l, t))
-STAGE(StBroadphase, if (m.ngeom <= 100) nxn_broadphase(m, d, t); else sap_broadphase(m, d, l, t))
+STAGE(StBroadphase, if (m.ngeom <= 100) nxn_broadphase(m, d, l, t); else sap_broadphase(m, d, l, t))
 STAGE(StNarrowphase, narrowphase(m, d, t))
-STAGE(StMakeConstraint, make_constraint(m, d, t))
-STAGE(StFwdVelocity, fwd_velocity(m, d, t))
+STAGE(StMakeConstraint, make_constraint(m, d, l, t))
+STAGE(StFwdVelocity, fwd_velocity(m, d, l, t))
 STAGE(StFwdActuation, fwd_actuation(m, d, t))
 STAGE(StFwdAcceleration, fwd_acceleration(m, d, l, t))
 STAGE(StEuler, euler(m, d, l, t))
@@ -102,16 +102,16 @@ struct StTail {
 struct StFusedStep {
   template <int T>
   MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
-    kinematics(m, d, t);
-    com_pos(m, d, t);
-    crb(m, d, t);
+    kinematics(m, d, l, t);
+    com_pos(m, d, l, t);
+    crb(m, d, l, t);
     factor_m(m, d, l, t);
     sync();
     collision(m, d, l, t);
-    make_constraint(m, d, t);
+    make_constraint(m, d, l, t);
     sync();
     transmission(m, d, t);
-    fwd_velocity(m, d, t);
+    fwd_velocity(m, d, l, t);
     fwd_actuation(m, d, t);
     fwd_acceleration(m, d, l, t);
     if (d.njmax == 0) {
@@ -137,15 +137,15 @@ struct StFusedStep {
 struct StFusedForward {
   template <int T>
   MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune&) const {
-    kinematics(m, d, t);
-    com_pos(m, d, t);
-    crb(m, d, t);
+    kinematics(m, d, l, t);
+    com_pos(m, d, l, t);
+    crb(m, d, l, t);
     sync();
     collision(m, d, l, t);
-    make_constraint(m, d, t);
+    make_constraint(m, d, l, t);
     sync();
     transmission(m, d, t);
-    fwd_velocity(m, d, t);
+    fwd_velocity(m, d, l, t);
     fwd_actuation(m, d, t);
     fwd_acceleration_force(m, d, t);
   }
@@ -235,9 +235,25 @@ int lds_solve(const mjhModel& m) {
   const int a = lds_solver_floats(m.nv, t.rmax, t.rj), b = lds_linalg(m);
   return a > b ? a : b;
 }
+int lds_kin(const mjhModel& m) { return lds_kinematics_floats(m.nbody); }
+int lds_crb(const mjhModel& m) { return lds_crb_floats(m.nbody, m.nv); }
+int lds_vel(const mjhModel& m) { return lds_velocity_floats(m.nbody); }
+int lds_broadphase(const mjhModel& m) { return m.ngeom > 100 ? lds_sap(m) : kNxnWords * m.ngeom; }
+bool reg_path(const mjhModel& m);
+int lds_con(const mjhModel& m) {
+  return reg_path(m) ? lds_constraint_floats<kRegTeam>() : lds_constraint_floats<kTeam>();
+}
+// every forward stage of the fused forward launch
+int lds_forward(const mjhModel& m) {
+  int a = lds_kin(m);
+  const int b = lds_crb(m), c = lds_vel(m), g = lds_broadphase(m), k = lds_con(m);
+  a = a > b ? a : b;
+  a = a > c ? a : c;
+  a = a > k ? a : k;
+  return a > g ? a : g;
+}
 int lds_step(const mjhModel& m) {
-  int a = lds_solve(m);
-  const int b = m.ngeom > 100 ? lds_sap(m) : 0;
+  const int a = lds_solve(m), b = lds_forward(m);
   return a > b ? a : b;
 }
 
@@ -297,7 +313,7 @@ int reset(const mjhData& d, hipStream_t s, int which) {
   } while (0)
 
 int do_collision(const mjhModel& m, const mjhData& d, hipStream_t s) {
-  TRY(launch<StBroadphase>(m, d, s, m.ngeom > 100 ? lds_sap(m) : 0, "broadphase"));
+  TRY(launch<StBroadphase>(m, d, s, lds_broadphase(m), "broadphase"));
   return launch<StNarrowphase>(m, d, s, 0, "narrowphase");
 }
 
@@ -308,22 +324,22 @@ int do_solve(const mjhModel& m, const mjhData& d, hipStream_t s) {
 }
 
 int do_fwd_position(const mjhModel& m, const mjhData& d, hipStream_t s) {
-  TRY(launch<StKinematics>(m, d, s, 0, "kinematics"));
-  TRY(launch<StComPos>(m, d, s, 0, "com_pos"));
-  TRY(launch<StCrb>(m, d, s, 0, "crb"));
+  TRY(launch<StKinematics>(m, d, s, lds_kin(m), "kinematics"));
+  TRY(launch<StComPos>(m, d, s, lds_kin(m), "com_pos"));
+  TRY(launch<StCrb>(m, d, s, lds_crb(m), "crb"));
   TRY(launch<StFactorM>(m, d, s, m.nv * m.nv, "factor_m"));
   TRY(reset(d, s, 1));
   TRY(do_collision(m, d, s));
   if (!(m.opt.disableflags & mjhDSBL_CONSTRAINT)) {
     TRY(reset(d, s, 2));
-    TRY(launch<StMakeConstraint>(m, d, s, 0, "make_constraint"));
+    TRY(launch<StMakeConstraint>(m, d, s, lds_con(m), "make_constraint"));
   }
   return launch<StTransmission>(m, d, s, 0, "transmission");
 }
 
 int do_forward(const mjhModel& m, const mjhData& d, hipStream_t s) {
   TRY(do_fwd_position(m, d, s));
-  TRY(launch<StFwdVelocity>(m, d, s, 0, "fwd_velocity"));
+  TRY(launch<StFwdVelocity>(m, d, s, lds_vel(m), "fwd_velocity"));
   TRY(launch<StFwdActuation>(m, d, s, 0, "fwd_actuation"));
   TRY(launch<StFwdAcceleration>(m, d, s, lds_linalg(m), "fwd_acceleration"));
   if (d.njmax == 0) return launch<StQaccSmooth>(m, d, s, 0, "qacc_smooth");
@@ -341,7 +357,7 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
     return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
   TRY(reset(d, s, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3));
   if (reg_path(m)) {
-    TRY((launch_team<StFusedForward, kRegTeam>(m, d, s, m.ngeom > 100 ? lds_sap(m) : 0, "step_fused_forward")));
+    TRY((launch_team<StFusedForward, kRegTeam>(m, d, s, lds_forward(m), "step_fused_forward")));
     TRY((launch_team<StFusedSolveIntegrate, kRegTeam>(m, d, s, lds_solve_reg(m), "step_fused_solve")));
   } else {
     TRY((launch_team<StFusedStep, kTeam>(m, d, s, lds_step(m), "step_fused")));
@@ -374,15 +390,15 @@ int mjh_check(const mjhModel* m, const mjhData* d) {
     return launch<S>(*m, *d, (hipStream_t)stream, (ldsexpr), #fname); \
   }
 
-STAGE_ENTRY(mjh_kinematics, StKinematics, 0)
-STAGE_ENTRY(mjh_com_pos, StComPos, 0)
-STAGE_ENTRY(mjh_crb, StCrb, 0)
+STAGE_ENTRY(mjh_kinematics, StKinematics, lds_kin(*m))
+STAGE_ENTRY(mjh_com_pos, StComPos, lds_kin(*m))
+STAGE_ENTRY(mjh_crb, StCrb, lds_crb(*m))
 STAGE_ENTRY(mjh_factor_m, StFactorM, m->nv * m->nv)
-STAGE_ENTRY(mjh_rne, StRne, 0)
+STAGE_ENTRY(mjh_rne, StRne, lds_vel(*m))
 STAGE_ENTRY(mjh_transmission, StTransmission, 0)
-STAGE_ENTRY(mjh_com_vel, StComVel, 0)
+STAGE_ENTRY(mjh_com_vel, StComVel, lds_vel(*m))
 STAGE_ENTRY(mjh_passive, StPassive, 0)
-STAGE_ENTRY(mjh_fwd_velocity, StFwdVelocity, 0)
+STAGE_ENTRY(mjh_fwd_velocity, StFwdVelocity, lds_vel(*m))
 STAGE_ENTRY(mjh_fwd_actuation, StFwdActuation, 0)
 STAGE_ENTRY(mjh_fwd_acceleration, StFwdAcceleration, lds_linalg(*m))
 
@@ -391,7 +407,7 @@ int mjh_solve(const mjhModel* m, const mjhData* d, mjhStream stream) {
 }
 int mjh_nxn_broadphase(const mjhModel* m, const mjhData* d, mjhStream stream) {
   TRY(reset(*d, (hipStream_t)stream, 1));
-  return launch<StNxn>(*m, *d, (hipStream_t)stream, 0, "nxn_broadphase");
+  return launch<StNxn>(*m, *d, (hipStream_t)stream, kNxnWords * m->ngeom, "nxn_broadphase");
 }
 int mjh_sap_broadphase(const mjhModel* m, const mjhData* d, mjhStream stream) {
   TRY(reset(*d, (hipStream_t)stream, 1));
@@ -407,7 +423,7 @@ int mjh_collision(const mjhModel* m, const mjhData* d, mjhStream stream) {
 int mjh_make_constraint(const mjhModel* m, const mjhData* d, mjhStream stream) {
   if (m->opt.disableflags & mjhDSBL_CONSTRAINT) return MJH_OK;
   TRY(reset(*d, (hipStream_t)stream, 2));
-  return launch<StMakeConstraint>(*m, *d, (hipStream_t)stream, 0, "make_constraint");
+  return launch<StMakeConstraint>(*m, *d, (hipStream_t)stream, lds_con(*m), "make_constraint");
 }
 int mjh_fwd_position(const mjhModel* m, const mjhData* d, mjhStream stream) {
   return do_fwd_position(*m, *d, (hipStream_t)stream);

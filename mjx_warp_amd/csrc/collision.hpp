@@ -29,11 +29,12 @@ MJH_DEV bool geom_filter(const mjhModel& m, int g1, int g2, bool filterparent) {
 }
 
 // upper-triangle pair index p -> (g1 < g2), lexicographic like
-// collision_driver.py:465-478, computed exactly in integers
+// collision_driver.py:465-478: a float estimate of the row, then exact
+// integer corrections
 MJH_DEV void pair_from_index(int p, int n, int* g1, int* g2) {
   // row r starts at r*(2n-r-1)/2
-  double disc = (double)(2 * n - 1) * (2 * n - 1) - 8.0 * p;
-  int r = (int)(((2 * n - 1) - sqrt(disc)) * 0.5);
+  const float disc = (float)(2 * n - 1) * (float)(2 * n - 1) - 8.0f * (float)p;
+  int r = (int)(((float)(2 * n - 1) - sqrtf(fmaxf(disc, 0.0f))) * 0.5f);
   if (r < 0) r = 0;
   while (r > 0 && (long)r * (2 * n - r - 1) / 2 > p) r--;
   while ((long)(r + 1) * (2 * n - r - 2) / 2 <= p) r++;
@@ -41,20 +42,54 @@ MJH_DEV void pair_from_index(int p, int n, int* g1, int* g2) {
   *g2 = p - r * (2 * n - r - 1) / 2 + r + 1;
 }
 
-// collision_driver.py:480-508 bounding test of the nxn broadphase
-MJH_DEV bool nxn_bounds(const mjhModel& m, const float* gxpos, const float* gxmat, int g1, int g2) {
-  const float m1 = m.geom_margin[g1], m2 = m.geom_margin[g2];
-  const V3 p1 = ld3(gxpos + 3 * g1), p2 = ld3(gxpos + 3 * g2);
-  const float s1 = m.geom_rbound[g1], s2 = m.geom_rbound[g2];
-  const float bound = s1 + s2 + fmaxf(m1, m2);
-  const V3 dif = p2 - p1;
-  if (s1 != 0.0f && s2 != 0.0f) return dot(dif, dif) <= bound * bound;
-  if (s1 == 0.0f) {
-    const float* x = gxmat + 9 * g1;
-    return dot(dif, V3{x[2], x[5], x[8]}) <= bound;
+// The nxn broadphase's per-geom inputs, staged in LDS once per world
+// (kNxnWords words per geom): the filter's ints (contype, conaffinity,
+// weld body, weld body of the weld parent) and the bounding test's floats
+// (position, rbound, plane normal = third column of geom_xmat, margin).
+constexpr int kNxnWords = 12;
+
+template <int TEAM>
+MJH_DEV void nxn_stage_geoms(const mjhModel& m, const float* gxpos, const float* gxmat, float* lds,
+                             const Team<TEAM>& t) {
+  int* gi = (int*)lds;
+  float* gf = lds + 4 * m.ngeom;
+  for (int g = t.lane; g < m.ngeom; g += TEAM) {
+    const int wd = m.body_weldid[m.geom_bodyid[g]];
+    gi[4 * g] = m.geom_contype[g];
+    gi[4 * g + 1] = m.geom_conaffinity[g];
+    gi[4 * g + 2] = wd;
+    gi[4 * g + 3] = m.body_weldid[m.body_parentid[wd]];
+    const float* x = gxmat + 9 * g;
+    gf[8 * g] = gxpos[3 * g];
+    gf[8 * g + 1] = gxpos[3 * g + 1];
+    gf[8 * g + 2] = gxpos[3 * g + 2];
+    gf[8 * g + 3] = m.geom_rbound[g];
+    gf[8 * g + 4] = x[2];
+    gf[8 * g + 5] = x[5];
+    gf[8 * g + 6] = x[8];
+    gf[8 * g + 7] = m.geom_margin[g];
   }
-  const float* x = gxmat + 9 * g2;
-  return dot(-dif, V3{x[2], x[5], x[8]}) <= bound;
+}
+
+// collision_driver.py:480-508 bounding test and collision_driver.py:27-49
+// filter of the nxn broadphase, on the staged geom table
+MJH_DEV bool nxn_accept(const float* lds, int ngeom, int g1, int g2, bool filterparent) {
+  const int* gi = (const int*)lds;
+  const float* gf = lds + 4 * ngeom;
+  const int ct1 = gi[4 * g1], ca1 = gi[4 * g1 + 1], w1 = gi[4 * g1 + 2], wp1 = gi[4 * g1 + 3];
+  const int ct2 = gi[4 * g2], ca2 = gi[4 * g2 + 1], w2 = gi[4 * g2 + 2], wp2 = gi[4 * g2 + 3];
+  const bool self = w1 == w2;
+  const bool parent_child = filterparent && w1 != 0 && w2 != 0 && (w1 == wp2 || w2 == wp1);
+  const bool mask = (ct1 & ca2) || (ct2 & ca1);
+  if (!mask || self || parent_child) return false;
+  const float* f1 = gf + 8 * g1;
+  const float* f2 = gf + 8 * g2;
+  const float s1 = f1[3], s2 = f2[3];
+  const float bound = s1 + s2 + fmaxf(f1[7], f2[7]);
+  const V3 dif = V3{f2[0], f2[1], f2[2]} - V3{f1[0], f1[1], f1[2]};
+  if (s1 != 0.0f && s2 != 0.0f) return dot(dif, dif) <= bound * bound;
+  if (s1 == 0.0f) return dot(dif, V3{f1[4], f1[5], f1[6]}) <= bound;
+  return dot(-dif, V3{f2[4], f2[5], f2[6]}) <= bound;
 }
 
 // ---- sweep-and-prune helpers (collision_driver.py:71-301) ----
@@ -113,26 +148,35 @@ struct PairWriter {
   }
 };
 
-// collision_driver.py:458-512 nxn_broadphase.  lds: >= 1 int per team slot.
+// collision_driver.py:458-512 nxn_broadphase.  lds: kNxnWords * ngeom
+// floats.  Every pair is tested once: bit i of `hits` keeps the result of
+// the lane's i-th pair for the write pass (pairs past the 32nd are tested
+// again there).
 template <int TEAM>
-MJH_DEV void nxn_broadphase(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+MJH_DEV void nxn_broadphase(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int w = t.world, ng = m.ngeom;
   const int npair = ng * (ng - 1) / 2;
   const bool fp = !(m.opt.disableflags & mjhDSBL_FILTERPARENT);
   const float* gxpos = WPTR(d.geom_xpos, w, ng * 3);
   const float* gxmat = WPTR(d.geom_xmat, w, ng * 9);
-  // pass 1: count
+  if (t.valid) nxn_stage_geoms(m, gxpos, gxmat, lds, t);
+  sync();
+  // pass 1: test and count
   int count = 0;
+  unsigned hits = 0;
   if (t.valid)
-    for (int p = t.lane; p < npair; p += TEAM) {
+    for (int p = t.lane, i = 0; p < npair; p += TEAM, i++) {
       int g1, g2;
       pair_from_index(p, ng, &g1, &g2);
-      if (nxn_bounds(m, gxpos, gxmat, g1, g2) && geom_filter(m, g1, g2, fp)) count++;
+      if (nxn_accept(lds, ng, g1, g2, fp)) {
+        count++;
+        if (i < 32) hits |= 1u << i;
+      }
     }
   const int total = team_sum_int<TEAM>(count);
   int base = 0;
   if (t.valid && t.lane == 0 && total > 0) base = atomicAdd(d.ncollision, total);
-  base = __shfl(base, 0, TEAM);
+  base = team_first<TEAM>(base);
   if (!t.valid) return;
   const int cap = d.nconmax;
   if (t.lane == 0) {
@@ -142,16 +186,16 @@ MJH_DEV void nxn_broadphase(const mjhModel& m, const mjhData& d, const Team<TEAM
   }
   // pass 2: write in pair order
   int k = 0;
-  for (int p0 = 0; p0 < npair; p0 += TEAM) {
+  for (int p0 = 0, i = 0; p0 < npair; p0 += TEAM, i++) {
     const int p = p0 + t.lane;
     int g1 = 0, g2 = 0;
     bool ok = false;
     if (p < npair) {
       pair_from_index(p, ng, &g1, &g2);
-      ok = nxn_bounds(m, gxpos, gxmat, g1, g2) && geom_filter(m, g1, g2, fp);
+      ok = i < 32 ? ((hits >> i) & 1u) != 0 : nxn_accept(lds, ng, g1, g2, fp);
     }
     int tot;
-    const int off = team_exscan_int<TEAM>(ok ? 1 : 0, t.lane, &tot);
+    const int off = team_exscan_bits<TEAM, 1>(ok ? 1 : 0, &tot);
     if (ok) {
       PairWriter<TEAM> pw{m, d, base, cap, k + off};
       pw.put(g1, g2, w);
@@ -273,7 +317,7 @@ MJH_DEV void sap_broadphase(const mjhModel& m, const mjhData& d, float* lds, con
   const int total = team_sum_int<TEAM>(count);
   int base = 0;
   if (t.valid && t.lane == 0 && total > 0) base = atomicAdd(d.ncollision, total);
-  base = __shfl(base, 0, TEAM);
+  base = team_first<TEAM>(base);
   const int cap = d.nconmax;
   if (t.valid && t.lane == 0) {
     d.world_pair[2 * w] = base;
@@ -428,9 +472,9 @@ MJH_DEV void narrowphase_pair(const mjhModel& m, const float* gxpos, const float
   // sphere-capsule and other pairs produce no contacts in the reference
 }
 
-// collision_driver.py:304-347 contact parameter mixing for contact idx
-MJH_DEV void contact_params(const mjhModel& m, const mjhData& d, int idx) {
-  const int g1 = d.contact_geom[2 * idx], g2 = d.contact_geom[2 * idx + 1];
+// collision_driver.py:304-347 contact parameter mixing of the contacts
+// [idx, idx + n) of the geom pair (g1, g2)
+MJH_DEV void contact_params(const mjhModel& m, const mjhData& d, int g1, int g2, int idx, int n) {
   const float margin = fmaxf(m.geom_margin[g1], m.geom_margin[g2]);
   const float gap = fmaxf(m.geom_gap[g1], m.geom_gap[g2]);
   const float sm1 = m.geom_solmix[g1], sm2 = m.geom_solmix[g2];
@@ -441,10 +485,10 @@ MJH_DEV void contact_params(const mjhModel& m, const mjhData& d, int idx) {
   const int p1 = m.geom_priority[g1], p2 = m.geom_priority[g2];
   if (p1 != p2) mix = p1 > p2 ? 1.0f : 0.0f;
   const int c1 = m.geom_condim[g1], c2 = m.geom_condim[g2];
-  d.contact_dim[idx] = (p1 == p2) ? max(c1, c2) : (p1 > p2 ? c1 : c2);
+  const int dim = (p1 == p2) ? max(c1, c2) : (p1 > p2 ? c1 : c2);
   const float* sr1 = m.geom_solref + 2 * g1;
   const float* sr2 = m.geom_solref + 2 * g2;
-  float* sr = d.contact_solref + 2 * idx;
+  float sr[2];
   if (sr1[0] > 0.0f && sr2[0] > 0.0f) {
     sr[0] = mix * sr1[0] + (1.0f - mix) * sr2[0];
     sr[1] = mix * sr1[1] + (1.0f - mix) * sr2[1];
@@ -452,20 +496,28 @@ MJH_DEV void contact_params(const mjhModel& m, const mjhData& d, int idx) {
     sr[0] = fminf(sr1[0], sr2[0]);
     sr[1] = fminf(sr1[1], sr2[1]);
   }
-  d.contact_includemargin[idx] = margin - gap;
   const float* f1 = m.geom_friction + 3 * g1;
   const float* f2 = m.geom_friction + 3 * g2;
   const float fa = fmaxf(f1[0], f2[0]), fb = fmaxf(f1[1], f2[1]), fc = fmaxf(f1[2], f2[2]);
-  float* fr = d.contact_friction + 5 * idx;
-  fr[0] = fa;
-  fr[1] = fa;
-  fr[2] = fb;
-  fr[3] = fc;
-  fr[4] = fc;
   const float* si1 = m.geom_solimp + 5 * g1;
   const float* si2 = m.geom_solimp + 5 * g2;
-  float* si = d.contact_solimp + 5 * idx;
+  float si[5];
+#pragma unroll
   for (int k = 0; k < 5; k++) si[k] = mix * si1[k] + (1.0f - mix) * si2[k];
+  for (int c = idx; c < idx + n; c++) {
+    d.contact_dim[c] = dim;
+    d.contact_solref[2 * c] = sr[0];
+    d.contact_solref[2 * c + 1] = sr[1];
+    d.contact_includemargin[c] = margin - gap;
+    float* fr = d.contact_friction + 5 * c;
+    fr[0] = fa;
+    fr[1] = fa;
+    fr[2] = fb;
+    fr[3] = fc;
+    fr[4] = fc;
+#pragma unroll
+    for (int k = 0; k < 5; k++) d.contact_solimp[5 * c + k] = si[k];
+  }
 }
 
 // collision_primitive.py:271-274 primitive_narrowphase over this world's
@@ -480,16 +532,25 @@ MJH_DEV void narrowphase(const mjhModel& m, const mjhData& d, const Team<TEAM>& 
   }
   const float* gxpos = WPTR(d.geom_xpos, w, ng * 3);
   const float* gxmat = WPTR(d.geom_xmat, w, ng * 9);
+  // pass 1 counts the contacts of each pair; the first pair of each lane
+  // is kept in registers for the write pass
   int count = 0;
+  int c0 = 0, g01 = 0, g02 = 0;
   for (int p = t.lane; p < npair; p += TEAM) {
+    const int g1 = d.collision_pair[2 * (padr + p)], g2 = d.collision_pair[2 * (padr + p) + 1];
     ContactSink s{&d, 0, 0, 0, w, false, 0, 0};
-    narrowphase_pair(m, gxpos, gxmat, d.collision_pair[2 * (padr + p)], d.collision_pair[2 * (padr + p) + 1], s);
+    narrowphase_pair(m, gxpos, gxmat, g1, g2, s);
     count += s.k;
+    if (p == t.lane) {
+      c0 = s.k;
+      g01 = g1;
+      g02 = g2;
+    }
   }
   const int total = team_sum_int<TEAM>(count);
   int base = 0;
   if (t.valid && t.lane == 0 && total > 0) base = atomicAdd(d.ncon, total);
-  base = __shfl(base, 0, TEAM);
+  base = team_first<TEAM>(base);
   const int cap = d.nconmax;
   const int nwritten = total <= 0 ? 0 : max(0, min(total, cap - base));
   if (t.valid && t.lane == 0) {
@@ -502,30 +563,36 @@ MJH_DEV void narrowphase(const mjhModel& m, const mjhData& d, const Team<TEAM>& 
     const int p = p0 + t.lane;
     int c = 0;
     int g1 = 0, g2 = 0;
-    if (p < npair) {
+    if (p0 == 0) {
+      c = c0;
+      g1 = g01;
+      g2 = g02;
+    } else if (p < npair) {
       g1 = d.collision_pair[2 * (padr + p)];
       g2 = d.collision_pair[2 * (padr + p) + 1];
       ContactSink s{&d, 0, 0, 0, w, false, 0, 0};
       narrowphase_pair(m, gxpos, gxmat, g1, g2, s);
       c = s.k;
     }
+    // a pair yields at most 4 contacts (plane-box)
     int tot;
-    const int off = team_exscan_int<TEAM>(c, t.lane, &tot);
+    const int off = team_exscan_bits<TEAM, 3>(c, &tot);
     if (c > 0) {
       ContactSink s{&d, base, cap, k + off, w, true, 0, 0};
       narrowphase_pair(m, gxpos, gxmat, g1, g2, s);
+      // contacts of this pair that fit in the pool
+      const int n = min(base + k + off + c, cap) - (base + k + off);
+      if (n > 0) contact_params(m, d, g1, g2, base + k + off, n);
     }
     k += tot;
   }
-  sync();
-  for (int i = t.lane; i < nwritten; i += TEAM) contact_params(m, d, base + i);
   sync();
 }
 
 // collision_driver.py:525-547 collision (counters are reset by the caller)
 template <int TEAM>
 MJH_DEV void collision(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
-  if (m.ngeom <= 100) nxn_broadphase(m, d, t);
+  if (m.ngeom <= 100) nxn_broadphase(m, d, lds, t);
   else sap_broadphase(m, d, lds, t);
   sync();
   narrowphase(m, d, t);

@@ -235,22 +235,93 @@ struct Team {
 // accesses of all lanes.
 MJH_DEV void sync() { __syncthreads(); }
 
-// Sum of v over the TEAM lanes of a team; every lane gets the same bits.
-// v is materialised (rounded) before the butterfly so that a product passed
-// in cannot be contracted into the first add differently in the two lanes
-// of a pair.
+template <int CTRL>
+MJH_DEV float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+MJH_DEV int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
+}
+
+// Sum over each 32-lane half of the wavefront (all lanes of a half get
+// identical bits: every step adds the same two rounded values in both lanes
+// of a pair).  The input is materialised first, so a product passed in is
+// rounded before the butterfly: contracting it into the first add as
+// fma(a, b, partner) would give the two lanes of a pair different bits, and
+// team-wide decisions taken on the sum would diverge.
+MJH_DEV float sum32(float v) {
+  asm volatile("" : "+v"(v));
+  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+  v += dpp_f<0x141>(v);  // row_half_mirror: 8-lane halves of a row
+  v += dpp_f<0x140>(v);  // row_mirror: the two 8-lane halves of a row
+  // bitmask swizzle inside 32-lane groups: and 0x1f, or 0, xor 0x10
+  v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+  return v;
+}
+MJH_DEV int sum32_int(int v) {
+  v += dpp_i<0xB1>(v);
+  v += dpp_i<0x4E>(v);
+  v += dpp_i<0x141>(v);
+  v += dpp_i<0x140>(v);
+  v += __builtin_amdgcn_ds_swizzle(v, 0x401F);
+  return v;
+}
+
+// Sum of v over the TEAM lanes of a team; every lane gets the same bits
+// (64-lane teams: the two half sums, read as wave-uniform values, added in
+// one order).
 template <int TEAM>
 MJH_DEV float team_sum(float v) {
-  asm volatile("" : "+v"(v));
-#pragma unroll
-  for (int o = TEAM / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, TEAM);
-  return v;
+  static_assert(TEAM == 32 || TEAM == 64, "team size");
+  const float s = sum32(v);
+  if constexpr (TEAM == 32) {
+    return s;
+  } else {
+    const float lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), 0));
+    const float hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), 32));
+    return lo + hi;
+  }
 }
 template <int TEAM>
 MJH_DEV int team_sum_int(int v) {
+  static_assert(TEAM == 32 || TEAM == 64, "team size");
+  const int s = sum32_int(v);
+  if constexpr (TEAM == 32) return s;
+  else return __builtin_amdgcn_readlane(s, 0) + __builtin_amdgcn_readlane(s, 32);
+}
+
+// v of lane 0 of this lane's team
+template <int TEAM>
+MJH_DEV int team_first(int v) {
+  const int lo = __builtin_amdgcn_readlane(v, 0);
+  if constexpr (TEAM == 64) return lo;
+  else return (threadIdx.x & 32) ? __builtin_amdgcn_readlane(v, 32) : lo;
+}
+
+// This lane's team as a mask over the wavefront's lanes.
+template <int TEAM>
+MJH_DEV unsigned long long team_lanes() {
+  if constexpr (TEAM == 64) return ~0ull;
+  else return 0xffffffffull << (threadIdx.x & 32);
+}
+
+// Exclusive prefix sum over the team (lane order) of v in [0, 2^BITS) and
+// the team's total, from one ballot per bit.
+template <int TEAM, int BITS>
+MJH_DEV int team_exscan_bits(int v, int* total) {
+  const unsigned long long team = team_lanes<TEAM>();
+  const unsigned long long below = __lanemask_lt() & team;
+  int off = 0, tot = 0;
 #pragma unroll
-  for (int o = TEAM / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, TEAM);
-  return v;
+  for (int b = 0; b < BITS; b++) {
+    const unsigned long long m = __ballot((v >> b) & 1) & team;
+    off += __popcll(m & below) << b;
+    tot += __popcll(m) << b;
+  }
+  *total = tot;
+  return off;
 }
 // Exclusive prefix sum over the TEAM lanes (lane order); *total gets the sum.
 template <int TEAM>

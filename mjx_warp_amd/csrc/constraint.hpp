@@ -47,9 +47,18 @@ MJH_DEV bool limit_active(const mjhModel& m, const float* qpos, int j, float* po
   return *pos < 0.0f;
 }
 
+// LDS floats per world of make_constraint: per contact of a chunk of TEAM
+// contacts, 5 ints (pool index, row offset, two bodies, compacted list) and
+// 20 floats (two offsets, frame, two frictions, distance, invweight), and
+// the J * qvel of its four rows
+template <int TEAM>
+__host__ __device__ constexpr int lds_constraint_floats() {
+  return 29 * TEAM;
+}
+
 // constraint.py:213-240 make_constraint (nefc is reset by the caller)
 template <int TEAM>
-MJH_DEV void make_constraint(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+MJH_DEV void make_constraint(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   if (m.opt.disableflags & mjhDSBL_CONSTRAINT) return;
   const int w = t.world, nv = m.nv;
   const bool refsafe = !(m.opt.disableflags & mjhDSBL_REFSAFE);
@@ -80,7 +89,7 @@ MJH_DEV void make_constraint(const mjhModel& m, const mjhData& d, const Team<TEA
   const int total = nlim + 4 * ncact;
   int base = 0;
   if (t.valid && t.lane == 0 && total > 0) base = atomicAdd(d.nefc, total);
-  base = __shfl(base, 0, TEAM);
+  base = team_first<TEAM>(base);
   const int cap = d.njmax;
   const int nwritten = total <= 0 ? 0 : max(0, min(total, cap - base));
   if (!t.valid) return;
@@ -102,7 +111,7 @@ MJH_DEV void make_constraint(const mjhModel& m, const mjhData& d, const Team<TEA
         act = limit_active(m, qpos, j, &pos, &J);
       }
       int tot;
-      const int off = team_exscan_int<TEAM>(act ? 1 : 0, t.lane, &tot);
+      const int off = team_exscan_bits<TEAM, 1>(act ? 1 : 0, &tot);
       if (act && k0 + off < nwritten) {
         const int r = base + k0 + off;
         d.efc_worldid[r] = w;
@@ -115,63 +124,108 @@ MJH_DEV void make_constraint(const mjhModel& m, const mjhData& d, const Team<TEA
       }
       k0 += tot;
     }
-  // contact rows (constraint.py:144-210), one contact at a time, lanes over dofs
-  int r0 = base + nlim;
+  // contact rows (constraint.py:144-210), in chunks of TEAM contacts: the
+  // per-contact scalars are gathered with lanes over contacts, the Jacobian
+  // rows are built one contact at a time with lanes over dofs, and the
+  // impedance / reference terms are computed with lanes over rows.  A
+  // contact's four rows are written only if all four fit.
+  int* lci = (int*)lds;
+  int* lrr = lci + TEAM;
+  int* lb = lrr + TEAM;
+  int* lact = lb + 2 * TEAM;
+  float* lf = (float*)(lact + TEAM);
+  float* ljq = lf + 20 * TEAM;
   const float* cdof = WPTR(d.cdof, w, nv * 6);
   const float* sc = WPTR(d.subtree_com, w, m.nbody * 3);
-  for (int c = 0; c < ncon; c++) {
-    const int ci = cadr + c;
-    const float pos = d.contact_dist[ci] - d.contact_includemargin[ci];
-    if (!(d.contact_dim[ci] == 3 && pos < 0.0f)) continue;
-    if (r0 - base + 4 > nwritten) break;
-    const int b1 = m.geom_bodyid[d.contact_geom[2 * ci]];
-    const int b2 = m.geom_bodyid[d.contact_geom[2 * ci + 1]];
-    const float* fr = d.contact_friction + 5 * ci;
-    const float fri0 = fr[0];
-    float invweight = m.body_invweight0[2 * b1] + m.body_invweight0[2 * b2];
-    invweight = invweight + fri0 * fri0 * invweight;
-    invweight = invweight * 2.0f * fri0 * fri0 / m.opt.impratio;
-    const M33 frame = ld9(d.contact_frame + 9 * ci);
-    const V3 cpos = ld3(d.contact_pos + 3 * ci);
-    const V3 off1 = cpos - ld3(sc + 3 * m.body_rootid[b1]);
-    const V3 off2 = cpos - ld3(sc + 3 * m.body_rootid[b2]);
-    const V3 n = V3{frame.m[0], frame.m[1], frame.m[2]};
-    const V3 t1 = V3{frame.m[3], frame.m[4], frame.m[5]};
-    const V3 t2 = V3{frame.m[6], frame.m[7], frame.m[8]};
-    float jq[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int i = t.lane; i < nv; i += TEAM) {
-      const S6 cd = ld6(cdof + 6 * i);
-      const V3 rot = V3{cd.v[0], cd.v[1], cd.v[2]}, lin = V3{cd.v[3], cd.v[4], cd.v[5]};
-      V3 jac1 = lin + cross(rot, off1);
-      V3 jac2 = lin + cross(rot, off2);
-      if (!m.body_dof_tree[b1 * nv + i]) jac1 = V3{0.f, 0.f, 0.f};
-      if (!m.body_dof_tree[b2 * nv + i]) jac2 = V3{0.f, 0.f, 0.f};
-      const V3 jd = jac2 - jac1;
-      const float d0 = dot(n, jd), d1 = dot(t1, jd), d2 = dot(t2, jd);
-      const float J0 = d0 + d1 * fr[0];
-      const float J1 = d0 - d1 * fr[0];
-      const float J2 = d0 + d2 * fr[1];
-      const float J3 = d0 - d2 * fr[1];
-      d.efc_J[(size_t)(r0 + 0) * nv + i] = J0;
-      d.efc_J[(size_t)(r0 + 1) * nv + i] = J1;
-      d.efc_J[(size_t)(r0 + 2) * nv + i] = J2;
-      d.efc_J[(size_t)(r0 + 3) * nv + i] = J3;
-      const float qv = qvel[i];
-      jq[0] += J0 * qv;
-      jq[1] += J1 * qv;
-      jq[2] += J2 * qv;
-      jq[3] += J3 * qv;
+  int rrun = nlim;  // rows of this world so far
+  for (int c0 = 0; c0 < ncon; c0 += TEAM) {
+    const int c = c0 + t.lane;
+    bool act = false;
+    if (c < ncon) {
+      const int ci = cadr + c;
+      const float pos = d.contact_dist[ci] - d.contact_includemargin[ci];
+      act = d.contact_dim[ci] == 3 && pos < 0.0f;
+      if (act) {
+        const int b1 = m.geom_bodyid[d.contact_geom[2 * ci]];
+        const int b2 = m.geom_bodyid[d.contact_geom[2 * ci + 1]];
+        const float* fr = d.contact_friction + 5 * ci;
+        const float fri0 = fr[0];
+        float invweight = m.body_invweight0[2 * b1] + m.body_invweight0[2 * b2];
+        invweight = invweight + fri0 * fri0 * invweight;
+        invweight = invweight * 2.0f * fri0 * fri0 / m.opt.impratio;
+        const V3 cpos = ld3(d.contact_pos + 3 * ci);
+        float* f = lf + 20 * t.lane;
+        st3(f, cpos - ld3(sc + 3 * m.body_rootid[b1]));
+        st3(f + 3, cpos - ld3(sc + 3 * m.body_rootid[b2]));
+        st9(f + 6, ld9(d.contact_frame + 9 * ci));
+        f[15] = fr[0];
+        f[16] = fr[1];
+        f[17] = pos;
+        f[18] = invweight;
+        lci[t.lane] = ci;
+        lb[2 * t.lane] = b1;
+        lb[2 * t.lane + 1] = b2;
+      }
     }
+    int nact;
+    const int off = team_exscan_bits<TEAM, 1>(act ? 1 : 0, &nact);
+    if (act) {
+      lact[off] = t.lane;
+      lrr[t.lane] = rrun + 4 * off;
+    }
+    sync();
+    for (int a = 0; a < nact; a++) {
+      const int k = lact[a];
+      const int rr = lrr[k];
+      if (rr + 4 > nwritten) break;
+      const int r0 = base + rr;
+      const int b1 = lb[2 * k], b2 = lb[2 * k + 1];
+      const float* f = lf + 20 * k;
+      const V3 off1 = ld3(f), off2 = ld3(f + 3);
+      const V3 n = ld3(f + 6), t1 = ld3(f + 9), t2 = ld3(f + 12);
+      const float fr0 = f[15], fr1 = f[16];
+      float jq[4] = {0.f, 0.f, 0.f, 0.f};
+      for (int i = t.lane; i < nv; i += TEAM) {
+        const S6 cd = ld6(cdof + 6 * i);
+        const V3 rot = V3{cd.v[0], cd.v[1], cd.v[2]}, lin = V3{cd.v[3], cd.v[4], cd.v[5]};
+        V3 jac1 = lin + cross(rot, off1);
+        V3 jac2 = lin + cross(rot, off2);
+        if (!m.body_dof_tree[b1 * nv + i]) jac1 = V3{0.f, 0.f, 0.f};
+        if (!m.body_dof_tree[b2 * nv + i]) jac2 = V3{0.f, 0.f, 0.f};
+        const V3 jd = jac2 - jac1;
+        const float d0 = dot(n, jd), d1 = dot(t1, jd), d2 = dot(t2, jd);
+        const float J0 = d0 + d1 * fr0;
+        const float J1 = d0 - d1 * fr0;
+        const float J2 = d0 + d2 * fr1;
+        const float J3 = d0 - d2 * fr1;
+        d.efc_J[(size_t)(r0 + 0) * nv + i] = J0;
+        d.efc_J[(size_t)(r0 + 1) * nv + i] = J1;
+        d.efc_J[(size_t)(r0 + 2) * nv + i] = J2;
+        d.efc_J[(size_t)(r0 + 3) * nv + i] = J3;
+        const float qv = qvel[i];
+        jq[0] += J0 * qv;
+        jq[1] += J1 * qv;
+        jq[2] += J2 * qv;
+        jq[3] += J3 * qv;
+      }
 #pragma unroll
-    for (int e = 0; e < 4; e++) jq[e] = team_sum<TEAM>(jq[e]);
-    if (t.lane < 4) {
-      const int r = r0 + t.lane;
-      d.efc_worldid[r] = w;
-      const float jqv = t.lane == 0 ? jq[0] : t.lane == 1 ? jq[1] : t.lane == 2 ? jq[2] : jq[3];
-      update_efc_row(m, d, r, pos, pos, invweight, d.contact_solref + 2 * ci, d.contact_solimp + 5 * ci,
-                     d.contact_includemargin[ci], refsafe, jqv);
+      for (int e = 0; e < 4; e++) jq[e] = team_sum<TEAM>(jq[e]);
+      if (t.lane == 0) st4(ljq + 4 * a, Q4{jq[0], jq[1], jq[2], jq[3]});
     }
-    r0 += 4;
+    sync();
+    for (int q = t.lane; q < 4 * nact; q += TEAM) {
+      const int k = lact[q >> 2];
+      const int rr = lrr[k];
+      if (rr + 4 > nwritten) continue;
+      const int r = base + rr + (q & 3);
+      const int ci = lci[k];
+      const float* f = lf + 20 * k;
+      d.efc_worldid[r] = w;
+      update_efc_row(m, d, r, f[17], f[17], f[18], d.contact_solref + 2 * ci, d.contact_solimp + 5 * ci,
+                     d.contact_includemargin[ci], refsafe, ljq[q]);
+    }
+    sync();
+    rrun += 4 * nact;
   }
 }
 

@@ -42,12 +42,12 @@ MJH_DEV void actuator_velocity(const mjhModel& m, const mjhData& d, const Team<T
 
 // forward.py:423-500 fwd_velocity
 template <int TEAM>
-MJH_DEV void fwd_velocity(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+MJH_DEV void fwd_velocity(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   actuator_velocity(m, d, t);
   sync();
-  com_vel(m, d, t);
+  com_vel(m, d, lds, t);
   passive(m, d, t);
-  rne(m, d, t);
+  rne(m, d, lds, t);
 }
 
 // forward.py:503-641 fwd_actuation

@@ -29,28 +29,6 @@ MJH_DEV void wave_fence() {
   asm volatile("" ::: "memory");
 }
 
-template <int CTRL>
-MJH_DEV float dpp_f(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
-}
-
-// Sum over the 32 lanes of a team (all lanes get identical bits: every
-// step adds the same two rounded values in both lanes of a pair).  The
-// input is materialised first, so a product passed in is rounded before the
-// butterfly: contracting it into the first add as fma(a, b, partner) would
-// give the two lanes of a pair different bits, and team-wide decisions
-// taken on the sum (line-search bracket, step length) would diverge.
-MJH_DEV float sum32(float v) {
-  asm volatile("" : "+v"(v));
-  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
-  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
-  v += dpp_f<0x141>(v);  // row_half_mirror: 8-lane halves of a row
-  v += dpp_f<0x140>(v);  // row_mirror: the two 8-lane halves of a row
-  // bitmask swizzle inside 32-lane groups: and 0x1f, or 0, xor 0x10
-  v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
-  return v;
-}
-
 // Value of v in lane k of this lane's team (k wave-uniform).
 MJH_DEV float bcast32(float v, int k) {
   const float lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));

@@ -6,6 +6,12 @@
 // reference stage cited in its comment; the reference's per-level launches
 // become per-level loops separated by sync(), and its atomic accumulations
 // become per-lane sequential sums (no two lanes write the same word).
+//
+// The tree passes keep the quantities they read back (parent poses, subtree
+// sums, composite inertias, body accelerations and forces) in the team's
+// LDS: each level or accumulation step then waits on an LDS access instead
+// of a global-memory round trip.  Data fields are still written to global
+// memory once each; the arithmetic and its order are unchanged.
 #pragma once
 
 #include "common.hpp"
@@ -17,10 +23,18 @@ MJH_DEV int level_end(const mjhModel& m, int l) {
   return (l == m.nlevel - 1) ? m.nbody : m.body_treeadr[l + 1];
 }
 
+// LDS floats per world of the tree passes below: kinematics (poses; also
+// enough for com_pos), crb (composite inertias, motion axes, dof parents
+// and the mass matrix), com_vel / rne (velocities, accelerations, forces)
+__host__ __device__ inline int lds_kinematics_floats(int nbody) { return 16 * nbody; }
+__host__ __device__ inline int lds_crb_floats(int nbody, int nv) { return 10 * nbody + 7 * nv + nv * nv; }
+__host__ __device__ inline int lds_velocity_floats(int nbody) { return 12 * nbody; }
+
 // ---------------------------------------------------------------------------
 // smooth.py:32-145 kinematics
+// lds: lds_kinematics_floats; holds xpos / xquat / xmat of the bodies done so far
 template <int TEAM>
-MJH_DEV void kinematics(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+MJH_DEV void kinematics(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int w = t.world;
   const int nbody = m.nbody;
   float* xpos = WPTR(d.xpos, w, nbody * 3);
@@ -31,6 +45,9 @@ MJH_DEV void kinematics(const mjhModel& m, const mjhData& d, const Team<TEAM>& t
   float* xanchor = WPTR(d.xanchor, w, m.njnt * 3);
   float* xaxis = WPTR(d.xaxis, w, m.njnt * 3);
   const float* qpos = WPTR(d.qpos, w, m.nq);
+  float* lpos = lds;
+  float* lquat = lds + 3 * nbody;
+  float* lmat = lds + 7 * nbody;
 
   if (t.valid && t.lane == 0) {
     st3(xpos, V3{0.f, 0.f, 0.f});
@@ -39,6 +56,9 @@ MJH_DEV void kinematics(const mjhModel& m, const mjhData& d, const Team<TEAM>& t
     M33 I = {{1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f}};
     st9(xmat, I);
     st9(ximat, I);
+    st3(lpos, V3{0.f, 0.f, 0.f});
+    st4(lquat, Q4{1.f, 0.f, 0.f, 0.f});
+    st9(lmat, I);
   }
   sync();
   for (int l = 1; l < m.nlevel; l++) {
@@ -57,8 +77,8 @@ MJH_DEV void kinematics(const mjhModel& m, const mjhData& d, const Team<TEAM>& t
         st3(xaxis + 3 * jntadr, ld3(m.jnt_axis + 3 * jntadr));
       } else {
         const int pid = m.body_parentid[b];
-        pos = mulmv(ld9(xmat + 9 * pid), ld3(m.body_pos + 3 * b)) + ld3(xpos + 3 * pid);
-        quat = mul_quat(ld4(xquat + 4 * pid), ld4(m.body_quat + 4 * b));
+        pos = mulmv(ld9(lmat + 9 * pid), ld3(m.body_pos + 3 * b)) + ld3(lpos + 3 * pid);
+        quat = mul_quat(ld4(lquat + 4 * pid), ld4(m.body_quat + 4 * b));
         for (int j = 0; j < jntnum; j++, jntadr++) {
           const int qadr = m.jnt_qposadr[jntadr];
           const int jt = m.jnt_type[jntadr];
@@ -80,9 +100,13 @@ MJH_DEV void kinematics(const mjhModel& m, const mjhData& d, const Team<TEAM>& t
         }
       }
       st3(xpos + 3 * b, pos);
+      st3(lpos + 3 * b, pos);
       quat = qnormalize(quat);
       st4(xquat + 4 * b, quat);
-      st9(xmat + 9 * b, quat_to_mat(quat));
+      st4(lquat + 4 * b, quat);
+      const M33 mat = quat_to_mat(quat);
+      st9(xmat + 9 * b, mat);
+      st9(lmat + 9 * b, mat);
       st3(xipos + 3 * b, pos + rot_vec_quat(ld3(m.body_ipos + 3 * b), quat));
       st9(ximat + 9 * b, quat_to_mat(mul_quat(quat, ld4(m.body_iquat + 4 * b))));
     }
@@ -93,16 +117,16 @@ MJH_DEV void kinematics(const mjhModel& m, const mjhData& d, const Team<TEAM>& t
     float* gxmat = WPTR(d.geom_xmat, w, m.ngeom * 9);
     for (int g = t.lane; g < m.ngeom; g += TEAM) {
       const int b = m.geom_bodyid[g];
-      const Q4 q = ld4(xquat + 4 * b);
-      st3(gxpos + 3 * g, ld3(xpos + 3 * b) + rot_vec_quat(ld3(m.geom_pos + 3 * g), q));
+      const Q4 q = ld4(lquat + 4 * b);
+      st3(gxpos + 3 * g, ld3(lpos + 3 * b) + rot_vec_quat(ld3(m.geom_pos + 3 * g), q));
       st9(gxmat + 9 * g, quat_to_mat(mul_quat(q, ld4(m.geom_quat + 4 * g))));
     }
     float* sxpos = WPTR(d.site_xpos, w, m.nsite * 3);
     float* sxmat = WPTR(d.site_xmat, w, m.nsite * 9);
     for (int s = t.lane; s < m.nsite; s += TEAM) {
       const int b = m.site_bodyid[s];
-      const Q4 q = ld4(xquat + 4 * b);
-      st3(sxpos + 3 * s, ld3(xpos + 3 * b) + rot_vec_quat(ld3(m.site_pos + 3 * s), q));
+      const Q4 q = ld4(lquat + 4 * b);
+      st3(sxpos + 3 * s, ld3(lpos + 3 * b) + rot_vec_quat(ld3(m.site_pos + 3 * s), q));
       st9(sxmat + 9 * s, quat_to_mat(mul_quat(q, ld4(m.site_quat + 4 * s))));
     }
   }
@@ -111,11 +135,13 @@ MJH_DEV void kinematics(const mjhModel& m, const mjhData& d, const Team<TEAM>& t
 
 // ---------------------------------------------------------------------------
 // smooth.py:148-245 com_pos
+// lds: 3 * nbody floats; holds the subtree sums / centres of mass
 template <int TEAM>
-MJH_DEV void com_pos(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+MJH_DEV void com_pos(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int w = t.world;
   const int nbody = m.nbody;
-  float* sc = WPTR(d.subtree_com, w, nbody * 3);
+  float* scw = WPTR(d.subtree_com, w, nbody * 3);
+  float* sc = lds;
   const float* xipos = WPTR(d.xipos, w, nbody * 3);
   if (t.valid)
     for (int b = t.lane; b < nbody; b += TEAM) st3(sc + 3 * b, ld3(xipos + 3 * b) * m.body_mass[b]);
@@ -128,7 +154,11 @@ MJH_DEV void com_pos(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
   }
   sync();
   if (t.valid)
-    for (int b = t.lane; b < nbody; b += TEAM) st3(sc + 3 * b, ld3(sc + 3 * b) * (1.0f / m.subtree_mass[b]));
+    for (int b = t.lane; b < nbody; b += TEAM) {
+      const V3 c = ld3(sc + 3 * b) * (1.0f / m.subtree_mass[b]);
+      st3(sc + 3 * b, c);
+      st3(scw + 3 * b, c);
+    }
   sync();
   if (t.valid) {
     const float* ximat = WPTR(d.ximat, w, nbody * 9);
@@ -202,14 +232,25 @@ MJH_DEV void com_pos(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
 
 // ---------------------------------------------------------------------------
 // smooth.py:248-314 crb (dense qM; the sparse layout is not built)
+// lds: lds_crb_floats; holds crb, cdof, dof_parentid and the mass matrix,
+// which is written to qM row by row at the end
 template <int TEAM>
-MJH_DEV void crb(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+MJH_DEV void crb(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int w = t.world;
   const int nbody = m.nbody, nv = m.nv;
   float* crbw = WPTR(d.crb, w, nbody * 10);
   const float* cinert = WPTR(d.cinert, w, nbody * 10);
-  if (t.valid)
-    for (int i = t.lane; i < nbody * 10; i += TEAM) crbw[i] = cinert[i];
+  const float* cdofw = WPTR(d.cdof, w, nv * 6);
+  float* lcrb = lds;
+  float* lcdof = lds + 10 * nbody;
+  int* lpar = (int*)(lcdof + 6 * nv);
+  float* lqM = lcdof + 7 * nv;
+  if (t.valid) {
+    for (int i = t.lane; i < nbody * 10; i += TEAM) lcrb[i] = cinert[i];
+    for (int i = t.lane; i < nv * 6; i += TEAM) lcdof[i] = cdofw[i];
+    for (int i = t.lane; i < nv; i += TEAM) lpar[i] = m.dof_parentid[i];
+    for (int i = t.lane; i < nv * nv; i += TEAM) lqM[i] = 0.0f;
+  }
   sync();
   // composite inertia: children into parents, bodies attached to the world
   // are not added to the world body (smooth.py:255-261)
@@ -217,26 +258,27 @@ MJH_DEV void crb(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
     const int k = t.lane;
     for (int b = nbody - 1; b >= 1; b--) {
       const int p = m.body_parentid[b];
-      if (p != 0) crbw[10 * p + k] += crbw[10 * b + k];
+      if (p != 0) lcrb[10 * p + k] += lcrb[10 * b + k];
     }
   }
-  float* qM = WPTR(d.qM, w, nv * nv);
-  if (t.valid)
-    for (int i = t.lane; i < nv * nv; i += TEAM) qM[i] = 0.0f;
   sync();
   if (t.valid) {
-    const float* cdof = WPTR(d.cdof, w, nv * 6);
+    for (int i = t.lane; i < nbody * 10; i += TEAM) crbw[i] = lcrb[i];
     for (int i = t.lane; i < nv; i += TEAM) {
-      const S6 ci = ld6(cdof + 6 * i);
-      const S6 buf = inert_vec(crbw + 10 * m.dof_bodyid[i], ci);
-      qM[i * nv + i] = m.dof_armature[i] + dot6(ci, buf);
-      for (int j = m.dof_parentid[i]; j >= 0; j = m.dof_parentid[j]) {
-        const float v = dot6(ld6(cdof + 6 * j), buf);
-        qM[i * nv + j] = v;
-        qM[j * nv + i] = v;
+      const S6 ci = ld6(lcdof + 6 * i);
+      const S6 buf = inert_vec(lcrb + 10 * m.dof_bodyid[i], ci);
+      lqM[i * nv + i] = m.dof_armature[i] + dot6(ci, buf);
+      for (int j = lpar[i]; j >= 0; j = lpar[j]) {
+        const float v = dot6(ld6(lcdof + 6 * j), buf);
+        lqM[i * nv + j] = v;
+        lqM[j * nv + i] = v;
       }
     }
   }
+  sync();
+  float* qM = WPTR(d.qM, w, nv * nv);
+  if (t.valid)
+    for (int i = t.lane; i < nv * nv; i += TEAM) qM[i] = lqM[i];
   sync();
 }
 
@@ -340,17 +382,22 @@ MJH_DEV void factor_m(const mjhModel& m, const mjhData& d, float* lds, const Tea
 
 // ---------------------------------------------------------------------------
 // smooth.py:398-466 rne
+// lds: lds_velocity_floats; holds the body accelerations and forces
 template <int TEAM>
-MJH_DEV void rne(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+MJH_DEV void rne(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int w = t.world;
   const int nbody = m.nbody, nv = m.nv;
-  float* cacc = WPTR(d.rne_cacc, w, nbody * 6);
-  float* cfrc = WPTR(d.rne_cfrc, w, nbody * 6);
+  float* caccw = WPTR(d.rne_cacc, w, nbody * 6);
+  float* cfrcw = WPTR(d.rne_cfrc, w, nbody * 6);
+  float* cacc = lds;
+  float* cfrc = lds + 6 * nbody;
   const float* cdof_dot = WPTR(d.cdof_dot, w, nv * 6);
   const float* qvel = WPTR(d.qvel, w, nv);
   if (t.valid && t.lane < 6) {
     const bool nograv = m.opt.disableflags & mjhDSBL_GRAVITY;
-    cacc[t.lane] = (t.lane < 3 || nograv) ? 0.0f : -m.opt.gravity[t.lane - 3];
+    const float g = (t.lane < 3 || nograv) ? 0.0f : -m.opt.gravity[t.lane - 3];
+    cacc[t.lane] = g;
+    caccw[t.lane] = g;
   }
   sync();
   for (int l = 0; l < m.nlevel; l++) {
@@ -366,6 +413,7 @@ MJH_DEV void rne(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
         for (int c = 0; c < 6; c++) a.v[c] += cdof_dot[6 * (da + i) + c] * qv;
       }
       st6(cacc + 6 * b, a);
+      st6(caccw + 6 * b, a);
     }
     sync();
   }
@@ -389,6 +437,7 @@ MJH_DEV void rne(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
   }
   sync();
   if (t.valid) {
+    for (int i = t.lane; i < nbody * 6; i += TEAM) cfrcw[i] = cfrc[i];
     const float* cdof = WPTR(d.cdof, w, nv * 6);
     float* bias = WPTR(d.qfrc_bias, w, nv);
     for (int i = t.lane; i < nv; i += TEAM) bias[i] = dot6(ld6(cdof + 6 * i), ld6(cfrc + 6 * m.dof_bodyid[i]));
@@ -449,15 +498,20 @@ MJH_DEV void transmission(const mjhModel& m, const mjhData& d, const Team<TEAM>&
 
 // ---------------------------------------------------------------------------
 // smooth.py:539-606 com_vel
+// lds: 6 * nbody floats; holds the body velocities
 template <int TEAM>
-MJH_DEV void com_vel(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
+MJH_DEV void com_vel(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int w = t.world;
   const int nbody = m.nbody, nv = m.nv;
-  float* cvel = WPTR(d.cvel, w, nbody * 6);
+  float* cvelw = WPTR(d.cvel, w, nbody * 6);
+  float* cvel = lds;
   float* cdof_dot = WPTR(d.cdof_dot, w, nv * 6);
   const float* cdof = WPTR(d.cdof, w, nv * 6);
   const float* qvel = WPTR(d.qvel, w, nv);
-  if (t.valid && t.lane < 6) cvel[t.lane] = 0.0f;
+  if (t.valid && t.lane < 6) {
+    cvel[t.lane] = 0.0f;
+    cvelw[t.lane] = 0.0f;
+  }
   sync();
   for (int l = 1; l < m.nlevel; l++) {
     const int beg = m.body_treeadr[l], end = level_end(m, l);
@@ -499,6 +553,7 @@ MJH_DEV void com_vel(const mjhModel& m, const mjhData& d, const Team<TEAM>& t) {
         }
       }
       st6(cvel + 6 * b, v);
+      st6(cvelw + 6 * b, v);
     }
     sync();
   }
