@@ -87,7 +87,7 @@ typedef struct mjhOption {
   X(nq) X(nv) X(nu) X(na) X(nbody) X(njnt) X(ngeom) X(nsite) X(nexclude) X(nmocap) \
   X(nM) X(nlsp) X(nlevel) X(nlimited) X(ntril) X(nfullm) X(nmulm) X(nqLD_update) \
   X(nqLD_update_level) X(nqLD_tile) X(nqLD_group) X(nact_tile) X(nact_group) \
-  X(nmesh) X(nmeshvert) X(actuator_affine_bias_gain) X(max_level_width)
+  X(nmesh) X(nmeshvert) X(actuator_affine_bias_gain) X(max_level_width) X(nnxn)
 
 #define MJH_MODEL_ARRAYS(X) \
   X(R, qpos0, (nq)) \
@@ -195,10 +195,19 @@ typedef struct mjhOption {
   X(R, actuator_actrange, (nu, 2)) \
   X(R, actuator_gear, (nu, 6)) \
   X(I, exclude_signature, (nexclude)) \
-  X(B, body_dof_tree, (nbody, nv))
+  X(B, body_dof_tree, (nbody, nv)) \
+  X(I, nxn_pair, (nnxn))
 /* body_dof_tree[b, i] = 1 iff dof i's body is an ancestor-or-self of body b
  * (b != 0): the kinematic-tree membership that constraint.py:86-93 and
- * support.py:152-158 recompute by walking body_parentid. */
+ * support.py:152-158 recompute by walking body_parentid.
+ * nxn_pair: for models of at most MJH_NXN_TABLE_MAX geoms, the geom pairs
+ * g1 < g2 in the nxn broadphase's order (collision_driver.py:465-478) that
+ * pass the model-only part of its filter (collision_driver.py:27-49:
+ * contype / conaffinity mask, distinct weld bodies), packed as
+ * g1 | g2 << 15 | parent_child << 30, where parent_child marks the pairs
+ * the parent filter removes unless DISABLE_FILTERPARENT is set; empty
+ * (nnxn = 0) for larger models. */
+#define MJH_NXN_TABLE_MAX 100
 
 /* ---- data ---- */
 #define MJH_DATA_SIZES(X) X(nworld) X(nconmax) X(njmax) X(qM_rows) X(qM_cols)

@@ -2,8 +2,8 @@
 // primitive narrowphase and contact-parameter mixing.
 //
 // Each world's broadphase pairs and contacts occupy one contiguous block of
-// the global pools (collision_pair / contact_*), reserved with a single
-// atomicAdd per world on ncollision / ncon.  Inside a block the order is the
+// the global pools (collision_pair / contact_*), reserved with one atomicAdd
+// per wavefront on ncollision / ncon.  Inside a block the order is the
 // world's pair enumeration order, so every world's contact set is
 // deterministic; only the placement of the blocks depends on scheduling,
 // as the reference's per-pair atomics do (collision_driver.py:54,
@@ -48,17 +48,20 @@ MJH_DEV void pair_from_index(int p, int n, int* g1, int* g2) {
 // (position, rbound, plane normal = third column of geom_xmat, margin).
 constexpr int kNxnWords = 12;
 
+// The filter's ints are only staged when `ints` is set (no pair table).
 template <int TEAM>
-MJH_DEV void nxn_stage_geoms(const mjhModel& m, const float* gxpos, const float* gxmat, float* lds,
+MJH_DEV void nxn_stage_geoms(const mjhModel& m, const float* gxpos, const float* gxmat, float* lds, bool ints,
                              const Team<TEAM>& t) {
   int* gi = (int*)lds;
   float* gf = lds + 4 * m.ngeom;
   for (int g = t.lane; g < m.ngeom; g += TEAM) {
-    const int wd = m.body_weldid[m.geom_bodyid[g]];
-    gi[4 * g] = m.geom_contype[g];
-    gi[4 * g + 1] = m.geom_conaffinity[g];
-    gi[4 * g + 2] = wd;
-    gi[4 * g + 3] = m.body_weldid[m.body_parentid[wd]];
+    if (ints) {
+      const int wd = m.body_weldid[m.geom_bodyid[g]];
+      gi[4 * g] = m.geom_contype[g];
+      gi[4 * g + 1] = m.geom_conaffinity[g];
+      gi[4 * g + 2] = wd;
+      gi[4 * g + 3] = m.body_weldid[m.body_parentid[wd]];
+    }
     const float* x = gxmat + 9 * g;
     gf[8 * g] = gxpos[3 * g];
     gf[8 * g + 1] = gxpos[3 * g + 1];
@@ -71,17 +74,10 @@ MJH_DEV void nxn_stage_geoms(const mjhModel& m, const float* gxpos, const float*
   }
 }
 
-// collision_driver.py:480-508 bounding test and collision_driver.py:27-49
-// filter of the nxn broadphase, on the staged geom table
-MJH_DEV bool nxn_accept(const float* lds, int ngeom, int g1, int g2, bool filterparent) {
-  const int* gi = (const int*)lds;
+// collision_driver.py:480-508 bounding test of the nxn broadphase, on the
+// staged geom table
+MJH_DEV bool nxn_bounds(const float* lds, int ngeom, int g1, int g2) {
   const float* gf = lds + 4 * ngeom;
-  const int ct1 = gi[4 * g1], ca1 = gi[4 * g1 + 1], w1 = gi[4 * g1 + 2], wp1 = gi[4 * g1 + 3];
-  const int ct2 = gi[4 * g2], ca2 = gi[4 * g2 + 1], w2 = gi[4 * g2 + 2], wp2 = gi[4 * g2 + 3];
-  const bool self = w1 == w2;
-  const bool parent_child = filterparent && w1 != 0 && w2 != 0 && (w1 == wp2 || w2 == wp1);
-  const bool mask = (ct1 & ca2) || (ct2 & ca1);
-  if (!mask || self || parent_child) return false;
   const float* f1 = gf + 8 * g1;
   const float* f2 = gf + 8 * g2;
   const float s1 = f1[3], s2 = f2[3];
@@ -90,6 +86,37 @@ MJH_DEV bool nxn_accept(const float* lds, int ngeom, int g1, int g2, bool filter
   if (s1 != 0.0f && s2 != 0.0f) return dot(dif, dif) <= bound * bound;
   if (s1 == 0.0f) return dot(dif, V3{f1[4], f1[5], f1[6]}) <= bound;
   return dot(-dif, V3{f2[4], f2[5], f2[6]}) <= bound;
+}
+
+// collision_driver.py:27-49 filter and the bounding test, on the staged
+// geom table
+MJH_DEV bool nxn_accept(const float* lds, int ngeom, int g1, int g2, bool filterparent) {
+  const int* gi = (const int*)lds;
+  const int ct1 = gi[4 * g1], ca1 = gi[4 * g1 + 1], w1 = gi[4 * g1 + 2], wp1 = gi[4 * g1 + 3];
+  const int ct2 = gi[4 * g2], ca2 = gi[4 * g2 + 1], w2 = gi[4 * g2 + 2], wp2 = gi[4 * g2 + 3];
+  const bool self = w1 == w2;
+  const bool parent_child = filterparent && w1 != 0 && w2 != 0 && (w1 == wp2 || w2 == wp1);
+  const bool mask = (ct1 & ca2) || (ct2 & ca1);
+  if (!mask || self || parent_child) return false;
+  return nxn_bounds(lds, ngeom, g1, g2);
+}
+
+// Candidate p of the nxn broadphase: entry p of the model's nxn_pair table
+// (model-only filter done) when the model has one, else upper-triangle
+// pair p with the full filter.  Sets (g1, g2) and returns whether the pair
+// passes; with `test` unset only sets (g1, g2).
+MJH_DEV bool nxn_candidate(const mjhModel& m, const float* lds, int p, bool table, bool fp, bool test, int* g1,
+                           int* g2) {
+  if (table) {
+    const int e = m.nxn_pair[p];
+    *g1 = e & 0x7fff;
+    *g2 = (e >> 15) & 0x7fff;
+    if (!test) return false;
+    if (fp && ((e >> 30) & 1)) return false;
+    return nxn_bounds(lds, m.ngeom, *g1, *g2);
+  }
+  pair_from_index(p, m.ngeom, g1, g2);
+  return test && nxn_accept(lds, m.ngeom, *g1, *g2, fp);
 }
 
 // ---- sweep-and-prune helpers (collision_driver.py:71-301) ----
@@ -159,24 +186,25 @@ MJH_DEV void nxn_broadphase(const mjhModel& m, const mjhData& d, float* lds, con
   const bool fp = !(m.opt.disableflags & mjhDSBL_FILTERPARENT);
   const float* gxpos = WPTR(d.geom_xpos, w, ng * 3);
   const float* gxmat = WPTR(d.geom_xmat, w, ng * 9);
-  if (t.valid) nxn_stage_geoms(m, gxpos, gxmat, lds, t);
+  // candidates: the model's filtered pair table, or every pair of a model
+  // too large for one
+  const bool table = ng <= MJH_NXN_TABLE_MAX;
+  const int ncand = table ? m.nnxn : npair;
+  if (t.valid) nxn_stage_geoms(m, gxpos, gxmat, lds, !table, t);
   sync();
   // pass 1: test and count
   int count = 0;
   unsigned hits = 0;
   if (t.valid)
-    for (int p = t.lane, i = 0; p < npair; p += TEAM, i++) {
+    for (int p = t.lane, i = 0; p < ncand; p += TEAM, i++) {
       int g1, g2;
-      pair_from_index(p, ng, &g1, &g2);
-      if (nxn_accept(lds, ng, g1, g2, fp)) {
+      if (nxn_candidate(m, lds, p, table, fp, true, &g1, &g2)) {
         count++;
         if (i < 32) hits |= 1u << i;
       }
     }
   const int total = team_sum_int<TEAM>(count);
-  int base = 0;
-  if (t.valid && t.lane == 0 && total > 0) base = atomicAdd(d.ncollision, total);
-  base = team_first<TEAM>(base);
+  const int base = team_reserve<TEAM>(d.ncollision, t.valid ? total : 0);
   if (!t.valid) return;
   const int cap = d.nconmax;
   if (t.lane == 0) {
@@ -186,13 +214,13 @@ MJH_DEV void nxn_broadphase(const mjhModel& m, const mjhData& d, float* lds, con
   }
   // pass 2: write in pair order
   int k = 0;
-  for (int p0 = 0, i = 0; p0 < npair; p0 += TEAM, i++) {
+  for (int p0 = 0, i = 0; p0 < ncand; p0 += TEAM, i++) {
     const int p = p0 + t.lane;
     int g1 = 0, g2 = 0;
     bool ok = false;
-    if (p < npair) {
-      pair_from_index(p, ng, &g1, &g2);
-      ok = i < 32 ? ((hits >> i) & 1u) != 0 : nxn_accept(lds, ng, g1, g2, fp);
+    if (p < ncand) {
+      ok = nxn_candidate(m, lds, p, table, fp, i >= 32, &g1, &g2);
+      if (i < 32) ok = ((hits >> i) & 1u) != 0;
     }
     int tot;
     const int off = team_exscan_bits<TEAM, 1>(ok ? 1 : 0, &tot);
@@ -315,9 +343,7 @@ MJH_DEV void sap_broadphase(const mjhModel& m, const mjhData& d, float* lds, con
       for (int j = i + 1; j <= i + range[i]; j++)
         if (geom_filter(m, val[i], val[j], fp) && sap_overlap(gs[i], gs[j])) count++;
   const int total = team_sum_int<TEAM>(count);
-  int base = 0;
-  if (t.valid && t.lane == 0 && total > 0) base = atomicAdd(d.ncollision, total);
-  base = team_first<TEAM>(base);
+  const int base = team_reserve<TEAM>(d.ncollision, t.valid ? total : 0);
   const int cap = d.nconmax;
   if (t.valid && t.lane == 0) {
     d.world_pair[2 * w] = base;
@@ -548,9 +574,7 @@ MJH_DEV void narrowphase(const mjhModel& m, const mjhData& d, const Team<TEAM>& 
     }
   }
   const int total = team_sum_int<TEAM>(count);
-  int base = 0;
-  if (t.valid && t.lane == 0 && total > 0) base = atomicAdd(d.ncon, total);
-  base = team_first<TEAM>(base);
+  const int base = team_reserve<TEAM>(d.ncon, t.valid ? total : 0);
   const int cap = d.nconmax;
   const int nwritten = total <= 0 ? 0 : max(0, min(total, cap - base));
   if (t.valid && t.lane == 0) {

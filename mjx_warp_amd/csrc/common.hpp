@@ -292,12 +292,18 @@ MJH_DEV int team_sum_int(int v) {
   else return __builtin_amdgcn_readlane(s, 0) + __builtin_amdgcn_readlane(s, 32);
 }
 
-// v of lane 0 of this lane's team
+// Reserves `total` consecutive entries of a global pool for this team and
+// returns the first: one atomicAdd on `counter` per wavefront (the teams of
+// a wavefront take adjacent blocks).  total is team-uniform (0 for teams
+// without a world); every lane of the wavefront must call it.
 template <int TEAM>
-MJH_DEV int team_first(int v) {
-  const int lo = __builtin_amdgcn_readlane(v, 0);
-  if constexpr (TEAM == 64) return lo;
-  else return (threadIdx.x & 32) ? __builtin_amdgcn_readlane(v, 32) : lo;
+MJH_DEV int team_reserve(int* counter, int total) {
+  const int t0 = __builtin_amdgcn_readlane(total, 0);
+  const int t1 = TEAM == 64 ? 0 : __builtin_amdgcn_readlane(total, 32);
+  int base = 0;
+  if ((threadIdx.x & 63) == 0 && t0 + t1 > 0) base = atomicAdd(counter, t0 + t1);
+  base = __builtin_amdgcn_readlane(base, 0);
+  return (TEAM == 32 && (threadIdx.x & 32)) ? base + t0 : base;
 }
 
 // This lane's team as a mask over the wavefront's lanes.

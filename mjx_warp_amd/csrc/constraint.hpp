@@ -1,6 +1,6 @@
 // Constraint rows of one world: joint limits (slide / hinge) and pyramidal
 // condim-3 contacts (constraint.py:22-240).  The world's rows form one
-// contiguous block of the efc pool (one atomicAdd on nefc per world):
+// contiguous block of the efc pool (one atomicAdd on nefc per wavefront):
 // limit rows in joint order, then four rows per active contact in contact
 // order, the same per-world order as the reference's two launches.
 #pragma once
@@ -87,9 +87,7 @@ MJH_DEV void make_constraint(const mjhModel& m, const mjhData& d, float* lds, co
   nlim = team_sum_int<TEAM>(nlim);
   ncact = team_sum_int<TEAM>(ncact);
   const int total = nlim + 4 * ncact;
-  int base = 0;
-  if (t.valid && t.lane == 0 && total > 0) base = atomicAdd(d.nefc, total);
-  base = team_first<TEAM>(base);
+  const int base = team_reserve<TEAM>(d.nefc, t.valid ? total : 0);
   const int cap = d.njmax;
   const int nwritten = total <= 0 ? 0 : max(0, min(total, cap - base));
   if (!t.valid) return;

@@ -163,7 +163,39 @@ def model_tables(mjm) -> dict:
     for i in range(nv):
       bdt[b, i] = mjm.dof_bodyid[i] in anc
   t["body_dof_tree"] = bdt
+  t["nxn_pair"] = nxn_pairs(mjm)
   return t
+
+
+NXN_TABLE_MAX = 100
+
+
+def nxn_pairs(mjm) -> np.ndarray:
+  """The nxn broadphase's candidate pairs (g1 < g2, lexicographic as
+  collision_driver.py:465-478) that pass the model-only part of the geom
+  filter (collision_driver.py:27-49: contype/conaffinity mask, distinct
+  weld bodies), packed g1 | g2 << 15 | parent_child << 30; parent_child
+  marks the pairs the parent filter drops unless it is disabled.  Empty for
+  models over NXN_TABLE_MAX geoms (the kernels then enumerate all pairs)."""
+  ng = int(mjm.ngeom)
+  if ng > NXN_TABLE_MAX:
+    return np.zeros(0, np.int32)
+  bodyid = np.asarray(mjm.geom_bodyid)
+  contype = np.asarray(mjm.geom_contype)
+  conaff = np.asarray(mjm.geom_conaffinity)
+  weld = np.asarray(mjm.body_weldid)
+  parent = np.asarray(mjm.body_parentid)
+  out = []
+  for g1 in range(ng):
+    for g2 in range(g1 + 1, ng):
+      w1, w2 = int(weld[bodyid[g1]]), int(weld[bodyid[g2]])
+      wp1, wp2 = int(weld[parent[w1]]), int(weld[parent[w2]])
+      mask = (int(contype[g1]) & int(conaff[g2])) or (int(contype[g2]) & int(conaff[g1]))
+      if not mask or w1 == w2:
+        continue
+      pc = w1 != 0 and w2 != 0 and (w1 == wp2 or w2 == wp1)
+      out.append(g1 | (g2 << 15) | (int(pc) << 30))
+  return np.asarray(out, np.int32)
 
 
 def put_model(mjm, device=None) -> types.Model:
@@ -202,6 +234,7 @@ def put_model(mjm, device=None) -> types.Model:
   m.nact_tile = len(tables["actuator_moment_offset_nv"])
   m.nact_group = len(tables["actuator_moment_tileadr"])
   m.max_level_width = int(tables["max_level_width"])
+  m.nnxn = len(tables["nxn_pair"])
   m.actuator_affine_bias_gain = int(
     bool(np.any(np.asarray(mjm.actuator_biastype) == 1) or np.any(np.asarray(mjm.actuator_gaintype) == 1)))
 

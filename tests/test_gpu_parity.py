@@ -348,6 +348,30 @@ def test_nxn_known_pairs_gpu():
   assert got == [[], [[0, 1]], [[0, 1], [0, 2], [1, 2]], [[3, 2]]]
 
 
+@pytest.mark.parametrize("filterparent", [True, False])
+def test_nxn_humanoid_pairs_match_oracle(filterparent):
+  """The nxn broadphase on the model's filtered pair table gives the oracle's
+  pairs, in the oracle's order, with the parent filter on and off (the
+  oracle enumerates every geom pair with the full filter)."""
+  mjm = models.humanoid()
+  if not filterparent:
+    mjm.opt.disableflags |= int(mjw.DisableBit.FILTERPARENT)
+  nworld = 8
+  q, _ = _perturbed(mjm, nworld, 11, scale=0.3, key=0)
+  m, d, hm, hd = _gpu_and_oracle(mjm, nworld, q, nconmax_w=256, njmax_w=4)
+  assert 0 < m.nnxn < mjm.ngeom * (mjm.ngeom - 1) // 2
+  mjw.kinematics(m, d)
+  mjw.nxn_broadphase(m, d)
+  torch.cuda.synchronize()
+  oracle.run("kinematics", hm, hd)
+  oracle.run("collision", hm, hd)
+  gp = helpers.world_blocks(helpers.data_numpy(d, "world_pair"), nworld)
+  op = helpers.world_blocks(hd["world_pair"], nworld)
+  assert [n for _, n in gp] == [n for _, n in op]
+  assert np.array_equal(helpers.gather_world_rows(helpers.data_numpy(d, "collision_pair"), gp),
+                        helpers.gather_world_rows(hd["collision_pair"], op))
+
+
 def test_sap_matches_oracle_random_scenes():
   from mjx_warp_amd import mjcf
 
