@@ -341,11 +341,13 @@ typedef struct mjhOption {
   X(I, overflow, (4))
 /* The last five arrays have no counterpart in types.py.  Each world's
  * broadphase pairs, contacts and constraint rows occupy one contiguous block
- * of the global pool, reserved with a single atomic per world; world_pair,
+ * of the global pool, reserved with one atomic per wavefront; world_pair,
  * world_con and world_efc hold (first index, count written) of that block so
  * that later stages find a world's rows without scanning the pool.
- * world_flags marks worlds whose rows exceed the LDS solver's row capacity;
- * a tail launch of the same step solves and integrates them.  overflow
+ * world_flags marks worlds whose rows exceed the solve launch's row
+ * capacity (64 rows for nv <= 32); tail launches of the same step solve and
+ * integrate them (the register solver up to 256 rows, then the
+ * global-memory solver).  overflow
  * counts pairs / contacts / rows dropped because a pool was full (the
  * reference drops contacts silently and writes rows out of bounds,
  * collision_primitive.py:64-65, constraint.py:119,162).
@@ -435,7 +437,7 @@ int mjh_step(const mjhModel* m, const mjhData* d, mjhStream stream);
 /* The same step as mjh_step with the stages fused: for nv <= 32 two
  * launches (every forward stage; solver and integrator) on 32-lane teams,
  * otherwise one launch per step; worlds whose constraint rows exceed the
- * fused solver's capacity finish in a tail launch.  No counterpart in the
+ * fused solver's capacity finish in tail launches.  No counterpart in the
  * reference; results equal mjh_step's up to floating-point summation
  * order. */
 int mjh_step_fused(const mjhModel* m, const mjhData* d, mjhStream stream);

@@ -40,6 +40,7 @@ from .io import is_sparse as is_sparse
 from .io import make_data as make_data
 from .io import put_data as put_data
 from .io import put_model as put_model
+from .test_util import benchmark as benchmark
 from .types import BiasType as BiasType
 from .types import ConeType as ConeType
 from .types import Contact as Contact

@@ -77,10 +77,29 @@ STAGE(StQaccSmooth, if (t.valid) for (int i = t.lane; i < m.nv; i += T) d.qacc[t
 // else the LDS solver; false when the world is left for the tail.
 template <int T>
 MJH_DEV bool solve_team(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) {
-  if constexpr (T == kRegTeam) return solve_reg(m, d, l, tune.rj, t);
+  if constexpr (T == kRegTeam) return solve_reg<2>(m, d, l, tune.rj, t);
   else return solve_lds(m, d, l, tune.rmax, tune.rj, t);
 }
 STAGE(StSolveTeam, if (!solve_team(m, d, l, t, tune) && t.lane == 0) d.world_flags[t.world] |= kFlagDeferred)
+// register tail (32-lane teams): flagged worlds of up to kRegRows rows get
+// the register solver with kRegSets row sets (and, when `integrate` is set,
+// the integrator), then the flag is cleared; larger worlds stay flagged
+template <bool integrate>
+struct StTailReg {
+  template <int T>
+  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
+    static_assert(T == kRegTeam, "register tail runs on 32-lane teams");
+    if (!t.valid || !(d.world_flags[t.world] & kFlagDeferred)) return;
+    if (!solve_reg<kRegSets>(m, d, l, tune.rj, t)) return;
+    sync();
+    if (integrate) {
+      if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
+      else implicit(m, d, l, t);
+    }
+    if (t.lane == 0) d.world_flags[t.world] &= ~kFlagDeferred;
+  }
+};
+
 // tail: flagged worlds get the global-memory solver (and, when `integrate`
 // is set, the integrator), then the flag is cleared
 template <bool integrate>
@@ -318,8 +337,12 @@ int do_collision(const mjhModel& m, const mjhData& d, hipStream_t s) {
 }
 
 int do_solve(const mjhModel& m, const mjhData& d, hipStream_t s) {
-  if (reg_path(m)) TRY((launch_team<StSolveTeam, kRegTeam>(m, d, s, lds_solve_reg(m), "solve")));
-  else TRY((launch_team<StSolveTeam, kTeam>(m, d, s, lds_solve(m), "solve")));
+  if (reg_path(m)) {
+    TRY((launch_team<StSolveTeam, kRegTeam>(m, d, s, lds_solve_reg(m), "solve")));
+    TRY((launch_team<StTailReg<false>, kRegTeam>(m, d, s, lds_solve_reg(m), "solve_tail_reg")));
+  } else {
+    TRY((launch_team<StSolveTeam, kTeam>(m, d, s, lds_solve(m), "solve")));
+  }
   return launch_team<StTail<false>, kTeam>(m, d, s, lds_linalg(m), "solve_tail");
 }
 
@@ -359,6 +382,7 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
   if (reg_path(m)) {
     TRY((launch_team<StFusedForward, kRegTeam>(m, d, s, lds_forward(m), "step_fused_forward")));
     TRY((launch_team<StFusedSolveIntegrate, kRegTeam>(m, d, s, lds_solve_reg(m), "step_fused_solve")));
+    TRY((launch_team<StTailReg<true>, kRegTeam>(m, d, s, lds_solve_reg(m), "step_tail_reg")));
   } else {
     TRY((launch_team<StFusedStep, kTeam>(m, d, s, lds_step(m), "step_fused")));
   }

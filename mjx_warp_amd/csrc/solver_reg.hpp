@@ -6,8 +6,9 @@
 // owns dof i -- its entries of qacc, Ma, grad, Mgrad, search, mv, the mass
 // matrix row (M is symmetric, so the row is read as a coalesced column) and
 // row i of the Newton Hessian and of its Cholesky factor, all in VGPRs.  Lane
-// r also owns constraint rows r and r + 32 (Jaref, jv, D, force and the
-// line-search quadratic).  The world's Jacobian rows are staged in LDS with
+// r also owns constraint rows r, r + 32, ..., r + 32 (RS - 1) (Jaref, jv, D,
+// force and the line-search quadratic): RS row sets, of which the first
+// ceil(nr / 32) are live.  The world's Jacobian rows are staged in LDS with
 // row stride 33, so both row-wise (lane = row) and column-wise (lane = dof)
 // reads are bank-conflict free; rows past the LDS capacity are read from
 // global memory, where they are read-only during the solve.
@@ -30,35 +31,37 @@
 
 namespace mjh {
 
-constexpr int kRegRows = 64;  // row capacity: two rows per lane
+constexpr int kRegSets = 8;                // most row sets per lane
+constexpr int kRegRows = 32 * kRegSets;    // row capacity of the register solver
 
-// LDS floats per world: broadcast buffers (vector 32, column 32, rows 64)
-// and rj staged Jacobian rows of stride 33, rounded up to a float4.
-__host__ __device__ inline int lds_reg_floats(int rj) { return (128 + rj * 33 + 3) & ~3; }
+// LDS floats per world: broadcast buffers (vector 32, column 32, rows
+// kRegRows) and rj staged Jacobian rows of stride 33, rounded up to a float4.
+__host__ __device__ inline int lds_reg_floats(int rj) { return (64 + kRegRows + rj * 33 + 3) & ~3; }
 
-template <int NVC>
+template <int NVC, int RS>
 struct RegSolver {
   static_assert(NVC % 4 == 0 && NVC <= kRegNvMax, "padded dof count");
+  static_assert(RS >= 1 && RS <= kRegSets, "row sets");
   const mjhModel& m;
   const mjhData& d;
   int w, nv, radr, nr, lane, rj;
   float* vbuf;  // [32] dof-vector broadcast
-  float* fbuf;  // [64] row-scalar broadcast
+  float* fbuf;  // [32 * RS] row-scalar broadcast
   float* Jl;    // [rj][33] staged Jacobian rows
   const float* M;  // qM of the world (global, read-only during the solve)
   // dof lane state
   RegChol<NVC> ch;  // Hessian row and its Cholesky factor (CG: factor of M)
   float qacc, Ma, grad, Mgrad, search, mv, qs, qas, qc;
-  // row lane state: rows lane and lane + 32
-  float Jaref[2], jv[2], D[2], force[2], q0[2], q1[2], q2[2];
+  // row lane state: rows lane + 32 s, s < RS
+  float Jaref[RS], jv[RS], D[RS], force[RS], q0[RS], q1[RS], q2[RS];
 
   int blk;   // CG: first dof of the lane's tree block
-  bool two;  // rows past 32 exist: the second row set is live (team-uniform)
+  int nset;  // live row sets, ceil(nr / 32) but at least 1 (team-uniform)
   // the factor in ch is current: Newton's H = M + J' D(active) J depends on
   // the iterate only through the active set, so while no row changes
   // activity the previous factor is the one a rebuild would produce
   bool fact_valid;
-  bool act_prev[2];
+  unsigned act_prev;  // bit s: row lane + 32 s was active at the last factorisation
 
   MJH_DEV int row(int s) const { return lane + 32 * s; }
   // global Jacobian row r (rows past the LDS capacity)
@@ -107,8 +110,8 @@ struct RegSolver {
   MJH_DEV void update_constraint(float* cost, float* gauss) {
     float c = 0.0f;
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
-      if (s == 1 && !two) break;
+    for (int s = 0; s < RS; s++) {
+      if (s >= nset) break;
       const float jar = Jaref[s];
       const bool a = jar < 0.0f;
       force[s] = a ? -1.0f * D[s] * jar : 0.0f;
@@ -116,8 +119,11 @@ struct RegSolver {
     }
     c = sum32(c);
     wave_fence();
-    fbuf[lane] = force[0];
-    fbuf[lane + 32] = force[1];
+#pragma unroll
+    for (int s = 0; s < RS; s++) {
+      if (s >= nset) break;
+      fbuf[lane + 32 * s] = force[s];
+    }
     wave_fence();
     float q = 0.0f;
     const int nl = nlds();
@@ -135,14 +141,18 @@ struct RegSolver {
     grad = Ma - qs - qc;
     const float gd = sum32(grad * grad);
     if (m.opt.solver == mjhSOL_NEWTON) {
-      const bool a0 = Jaref[0] < 0.0f, a1 = two && Jaref[1] < 0.0f;
-      const float changed = sum32((a0 != act_prev[0] || a1 != act_prev[1]) ? 1.0f : 0.0f);
+      unsigned act = 0;
+#pragma unroll
+      for (int s = 0; s < RS; s++) {
+        if (s >= nset) break;
+        if (Jaref[s] < 0.0f) act |= 1u << s;
+      }
+      const float changed = sum32(act != act_prev ? 1.0f : 0.0f);
       if (fact_valid && changed == 0.0f) {
         Mgrad = ch.solve(grad);
         return gd;
       }
-      act_prev[0] = a0;
-      act_prev[1] = a1;
+      act_prev = act;
       fact_valid = true;
       const int ln = opaque(lane);
       gfloat* Mc = opaque(M) + (ln < nv ? ln : 0);
@@ -152,8 +162,11 @@ struct RegSolver {
         ch.h[j] = (ln < nv && j < nv) ? v : (j == ln ? 1.0f : 0.0f);
       }
       wave_fence();
-      fbuf[lane] = Jaref[0] < 0.0f ? D[0] : 0.0f;
-      fbuf[lane + 32] = Jaref[1] < 0.0f ? D[1] : 0.0f;
+#pragma unroll
+      for (int s = 0; s < RS; s++) {
+        if (s >= nset) break;
+        fbuf[lane + 32 * s] = Jaref[s] < 0.0f ? D[s] : 0.0f;
+      }
       wave_fence();
       const int nl = nlds();
       for (int r = 0; r < nl; r++) {
@@ -186,8 +199,8 @@ struct RegSolver {
   MJH_DEV V3f rows_at(float alpha) const {
     float a = 0.f, b = 0.f, c = 0.f;
 #pragma unroll
-    for (int s = 0; s < 2; s++)
-      if (s == 0 || two)
+    for (int s = 0; s < RS; s++)
+      if (s < nset)
       if (Jaref[s] + alpha * jv[s] < 0.0f) {
         const V3f p = eval_pt(q0[s], q1[s], q2[s], alpha);
         a += p.a;
@@ -202,8 +215,8 @@ struct RegSolver {
     const float gtol = m.opt.tolerance * m.opt.ls_tolerance * sqrtf(search_dot) * m.meaninertia * (float)max(1, nv);
     float pa = 0.f, pb = 0.f, pc = 0.f;
 #pragma unroll
-    for (int s = 0; s < 2; s++)
-      if (s == 0 || two)
+    for (int s = 0; s < RS; s++)
+      if (s < nset)
       if (Jaref[s] < 0.0f) {
         pa += q0[s];
         pb += q1[s];
@@ -231,8 +244,8 @@ struct RegSolver {
       const float ma = 0.5f * (lo_alpha + hi_alpha);
       float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f, s8 = 0.f;
 #pragma unroll
-      for (int s = 0; s < 2; s++) {
-        if (s == 1 && !two) break;
+      for (int s = 0; s < RS; s++) {
+        if (s >= nset) break;
         const float jar = Jaref[s], v = jv[s];
         if (jar + la * v < 0.0f) {
           const V3f p = eval_pt(q0[s], q1[s], q2[s], la);
@@ -274,8 +287,8 @@ struct RegSolver {
       const float a = m.alpha_candidate[k];
       float s0 = 0.f, s1 = 0.f, s2 = 0.f;
 #pragma unroll
-      for (int s = 0; s < 2; s++)
-        if (s == 0 || two)
+      for (int s = 0; s < RS; s++)
+        if (s < nset)
         if (Jaref[s] + a * jv[s] < 0.0f) {
           s0 += q0[s];
           s1 += q1[s];
@@ -303,8 +316,8 @@ struct RegSolver {
     put_vec(search);
     mv = mrow_dot();
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
-      if (s == 1 && !two) break;
+    for (int s = 0; s < RS; s++) {
+      if (s >= nset) break;
       const float v = jrow_dot(row(s));
       const float jar = Jaref[s], dd = D[s];
       jv[s] = v;
@@ -320,28 +333,26 @@ struct RegSolver {
     qacc += alpha * search;
     Ma += alpha * mv;
 #pragma unroll
-    for (int s = 0; s < 2; s++)
-      if (s == 0 || two) Jaref[s] += alpha * jv[s];
+    for (int s = 0; s < RS; s++)
+      if (s < nset) Jaref[s] += alpha * jv[s];
     return alpha;
   }
 };
 
-// Solves world t.world with the register solver; returns false (and leaves
-// the world untouched) when its rows exceed the row capacity.
-template <int NVC>
-MJH_DEV bool solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int rj, const Team<kRegTeam>& t) {
+// Solves world t.world (at most 32 RS rows) with the register solver.
+template <int NVC, int RS>
+MJH_DEV void solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int rj, const Team<kRegTeam>& t) {
   const int w = t.world, nv = m.nv;
   const int radr = d.world_efc[2 * w], nr = d.world_efc[2 * w + 1];
-  if (nr > kRegRows) return false;
-  RegSolver<NVC> s{m, d, w, nv, radr, nr, t.lane, rj};
+  RegSolver<NVC, RS> s{m, d, w, nv, radr, nr, t.lane, rj};
   s.vbuf = lds;
   s.ch.lane = t.lane;
   s.ch.cbuf = lds + 32;
   s.fbuf = lds + 64;
-  s.Jl = lds + 128;
-  s.two = nr > 32;
+  s.Jl = lds + 64 + kRegRows;
+  s.nset = nr > 32 ? (nr + 31) / 32 : 1;
   s.fact_valid = false;
-  s.act_prev[0] = s.act_prev[1] = false;
+  s.act_prev = 0;
   const int lane = t.lane;
   const bool dof = lane < nv;
   s.ch.rdiag = 1.0f;
@@ -352,9 +363,9 @@ MJH_DEV bool solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int r
   s.qs = dof ? d.qfrc_smooth[o + lane] : 0.0f;
   s.qas = dof ? d.qacc_smooth[o + lane] : 0.0f;
   s.M = d.qM + o * nv;
-  float aref[2];
+  float aref[RS];
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < RS; k++) {
     const int r = s.row(k);
     s.D[k] = r < nr ? d.efc_D[radr + r] : 0.0f;
     s.jv[k] = 0.0f;
@@ -373,7 +384,7 @@ MJH_DEV bool solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int r
   // _create_context (solver.py:26-72)
   s.put_vec(s.qacc);
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < RS; k++) {
     const int r = s.row(k);
     float a = 0.0f;
     if (r < nr) {
@@ -431,7 +442,7 @@ MJH_DEV bool solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int r
     d.efc_mv[o + lane] = s.mv;
   }
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < RS; k++) {
     const int r = s.row(k);
     if (r < nr) {
       d.efc_force[radr + r] = s.force[k];
@@ -453,21 +464,21 @@ MJH_DEV bool solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int r
     d.efc_done[w] = done;
     d.efc_solver_niter[w] = 0;
   }
-  return true;
 }
 
+// The register solver for world t.world with RS row sets: returns false
+// (and leaves the world untouched) when its rows exceed 32 RS.  The solve
+// launches take RS = 2 (64 rows, the register budget of two wavefronts per
+// SIMD); a tail launch takes the larger worlds with RS = kRegSets.
+template <int RS>
 MJH_DEV bool solve_reg(const mjhModel& m, const mjhData& d, float* lds, int rj, const Team<kRegTeam>& t) {
   if (!t.valid) return true;
-  switch ((m.nv + 3) / 4) {
-    case 1: return solve_reg_nv<4>(m, d, lds, rj, t);
-    case 2: return solve_reg_nv<8>(m, d, lds, rj, t);
-    case 3: return solve_reg_nv<12>(m, d, lds, rj, t);
-    case 4: return solve_reg_nv<16>(m, d, lds, rj, t);
-    case 5: return solve_reg_nv<20>(m, d, lds, rj, t);
-    case 6: return solve_reg_nv<24>(m, d, lds, rj, t);
-    case 7: return solve_reg_nv<28>(m, d, lds, rj, t);
-    default: return solve_reg_nv<32>(m, d, lds, rj, t);
-  }
+  if (d.world_efc[2 * t.world + 1] > 32 * RS) return false;
+  with_nvc(m.nv, [&](auto nvc) {
+    solve_reg_nv<decltype(nvc)::value, RS>(m, d, lds, rj, t);
+    return 0;
+  });
+  return true;
 }
 
 }  // namespace mjh

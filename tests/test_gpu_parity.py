@@ -252,6 +252,33 @@ def test_step(model, fused):
   assert abs(d.time - mjm.opt.timestep) < 1e-12
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_step_lying_humanoid_many_rows(fused):
+  """A humanoid lying on the floor (torso at z = 0.05, on its back): 60-110
+  constraint rows per world.  Worlds over the solve launch's 64 rows are
+  solved (and, fused, integrated) by the register tail launch with eight
+  row sets per lane; the step is held to the oracle's as in test_step."""
+  mjm = models.humanoid(iterations=10, ls_iterations=20)
+  q0 = mjm.key_qpos[0].copy()
+  q0[:3] = [0.0, 0.0, 0.05]
+  q0[3:7] = [np.cos(np.pi / 4), 0.0, np.sin(np.pi / 4), 0.0]
+  q0[7:] = 0.0
+  mjm.key_qpos[0] = q0
+  args = dict(nworld=16, seed=4, nconmax_w=64, njmax_w=277, key=0)
+  m, d, hm, hd = make_case(mjm, **args)
+  (mjw.step_fused if fused else mjw.step)(m, d)
+  torch.cuda.synchronize()
+  oracle.run("step", hm, hd)
+  rows = np.array([n for _, n in helpers.world_blocks(hd["world_efc"], 16)])
+  assert rows.max() > 64 and rows.min() <= 64 and rows.max() <= 256, rows
+  assert np.array_equal(rows, [n for _, n in helpers.world_blocks(helpers.data_numpy(d, "world_efc"), 16)])
+  assert not np.any(helpers.data_numpy(d, "world_flags")), "a world was left deferred"
+  env = oracle_envelope(mjm, stage="step", fields=["qpos", "qvel", "qacc"], **args)
+  _close_or_enveloped("qpos", helpers.data_numpy(d, "qpos"), hd["qpos"], env["qpos"], 1e-4)
+  _close_or_enveloped("qvel", helpers.data_numpy(d, "qvel"), hd["qvel"], env["qvel"], 2e-3)
+  _close_or_enveloped("qacc", helpers.data_numpy(d, "qacc"), hd["qacc"], env["qacc"], 2e-3)
+
+
 def _gpu_and_oracle(mjm, nworld, qpos, qvel=None, ctrl=None, nconmax_w=32, njmax_w=128):
   out = []
   for dev in ("cuda", "cpu"):
