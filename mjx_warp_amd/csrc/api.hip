@@ -289,7 +289,20 @@ bool reg_path(const mjhModel& m) {
 // per-world LDS of the 32-lane kernels, a multiple of 4 floats so every
 // world's broadcast buffers are float4-aligned
 int lds_solve_reg(const mjhModel& m) {
-  const int a = lds_reg_floats(tune_for(m).rj), b = lds_linalg(m);
+  const int a = lds_reg_floats(tune_for(m).rj, (m.nv + 3) & ~3), b = lds_linalg(m);
+  return ((a > b ? a : b) + 3) & ~3;
+}
+// The register tail stages as many Jacobian rows as two worlds' LDS fit in
+// 64 KB (up to every row of kRegRows).
+Tune tune_tail_reg(const mjhModel& m) {
+  Tune t = tune_for(m);
+  const int nvc = (m.nv + 3) & ~3;
+  const int fit = (16384 / 2 - 64 - kRegRows - 3) / (nvc + 1);
+  t.rj = fit < kRegRows ? fit : kRegRows;
+  return t;
+}
+int lds_tail_reg(const mjhModel& m) {
+  const int a = lds_reg_floats(tune_tail_reg(m).rj, (m.nv + 3) & ~3), b = lds_linalg(m);
   return ((a > b ? a : b) + 3) & ~3;
 }
 
@@ -300,7 +313,8 @@ int check_launch(const char* what) {
 }
 
 template <class S, int TEAM>
-int launch_team(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, const char* name) {
+int launch_team(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, const char* name,
+                const Tune* tune = nullptr) {
   if (d.nworld <= 0) return MJH_OK;
   constexpr int wpb = 64 / TEAM;
   const int grid = (d.nworld + wpb - 1) / wpb;
@@ -308,7 +322,7 @@ int launch_team(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, co
   if (bytes > 65536)
     return fail(MJH_ERR_CAPACITY, std::string(name) + ": model too large for the dense LDS kernels (nv = " +
                                       std::to_string(m.nv) + ")");
-  hipLaunchKernelGGL((k_stage<TEAM, S>), dim3(grid), dim3(64), bytes, s, m, d, ldsw, tune_for(m));
+  hipLaunchKernelGGL((k_stage<TEAM, S>), dim3(grid), dim3(64), bytes, s, m, d, ldsw, tune ? *tune : tune_for(m));
   return check_launch(name);
 }
 
@@ -339,7 +353,8 @@ int do_collision(const mjhModel& m, const mjhData& d, hipStream_t s) {
 int do_solve(const mjhModel& m, const mjhData& d, hipStream_t s) {
   if (reg_path(m)) {
     TRY((launch_team<StSolveTeam, kRegTeam>(m, d, s, lds_solve_reg(m), "solve")));
-    TRY((launch_team<StTailReg<false>, kRegTeam>(m, d, s, lds_solve_reg(m), "solve_tail_reg")));
+    const Tune tt = tune_tail_reg(m);
+    TRY((launch_team<StTailReg<false>, kRegTeam>(m, d, s, lds_tail_reg(m), "solve_tail_reg", &tt)));
   } else {
     TRY((launch_team<StSolveTeam, kTeam>(m, d, s, lds_solve(m), "solve")));
   }
@@ -382,7 +397,8 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
   if (reg_path(m)) {
     TRY((launch_team<StFusedForward, kRegTeam>(m, d, s, lds_forward(m), "step_fused_forward")));
     TRY((launch_team<StFusedSolveIntegrate, kRegTeam>(m, d, s, lds_solve_reg(m), "step_fused_solve")));
-    TRY((launch_team<StTailReg<true>, kRegTeam>(m, d, s, lds_solve_reg(m), "step_tail_reg")));
+    const Tune tt = tune_tail_reg(m);
+    TRY((launch_team<StTailReg<true>, kRegTeam>(m, d, s, lds_tail_reg(m), "step_tail_reg", &tt)));
   } else {
     TRY((launch_team<StFusedStep, kTeam>(m, d, s, lds_step(m), "step_fused")));
   }

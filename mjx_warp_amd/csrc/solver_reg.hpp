@@ -9,9 +9,9 @@
 // r also owns constraint rows r, r + 32, ..., r + 32 (RS - 1) (Jaref, jv, D,
 // force and the line-search quadratic): RS row sets, of which the first
 // ceil(nr / 32) are live.  The world's Jacobian rows are staged in LDS with
-// row stride 33, so both row-wise (lane = row) and column-wise (lane = dof)
-// reads are bank-conflict free; rows past the LDS capacity are read from
-// global memory, where they are read-only during the solve.
+// the odd row stride NVC + 1, so both row-wise (lane = row) and column-wise
+// (lane = dof) reads are bank-conflict free; rows past the staged count are
+// read from global memory, where they are read-only during the solve.
 //
 // Cross-lane sums are DPP butterflies inside 16-lane rows plus one swizzle
 // across the two rows of a team; every lane of the team ends with the same
@@ -35,8 +35,9 @@ constexpr int kRegSets = 8;                // most row sets per lane
 constexpr int kRegRows = 32 * kRegSets;    // row capacity of the register solver
 
 // LDS floats per world: broadcast buffers (vector 32, column 32, rows
-// kRegRows) and rj staged Jacobian rows of stride 33, rounded up to a float4.
-__host__ __device__ inline int lds_reg_floats(int rj) { return (64 + kRegRows + rj * 33 + 3) & ~3; }
+// kRegRows) and rj staged Jacobian rows of stride nvc + 1 (nvc: nv rounded
+// up to 4), rounded up to a float4.
+__host__ __device__ inline int lds_reg_floats(int rj, int nvc) { return (64 + kRegRows + rj * (nvc + 1) + 3) & ~3; }
 
 template <int NVC, int RS>
 struct RegSolver {
@@ -47,7 +48,8 @@ struct RegSolver {
   int w, nv, radr, nr, lane, rj;
   float* vbuf;  // [32] dof-vector broadcast
   float* fbuf;  // [32 * RS] row-scalar broadcast
-  float* Jl;    // [rj][33] staged Jacobian rows
+  float* Jl;    // [rj][JS] staged Jacobian rows
+  static constexpr int JS = NVC + 1;
   const float* M;  // qM of the world (global, read-only during the solve)
   // dof lane state
   RegChol<NVC> ch;  // Hessian row and its Cholesky factor (CG: factor of M)
@@ -90,7 +92,7 @@ struct RegSolver {
     float s = 0.0f;
     if (r >= nr) return s;
     if (r < rj) {
-      const float* Jr = Jl + r * 33;
+      const float* Jr = Jl + r * JS;
 #pragma unroll
       for (int q = 0; q < NVC / 4; q++) {
         const float4 v = vec4(q);
@@ -127,7 +129,8 @@ struct RegSolver {
     wave_fence();
     float q = 0.0f;
     const int nl = nlds();
-    for (int r = 0; r < nl; r++) q += Jl[r * 33 + lane] * fbuf[r];
+    if (lane < NVC)
+      for (int r = 0; r < nl; r++) q += Jl[r * JS + lane] * fbuf[r];
     const int ci = lane < nv ? lane : 0;
     for (int r = nl; r < nr; r++) q += (lane < nv ? Jg(r)[ci] : 0.0f) * fbuf[r];
     qc = q;
@@ -172,8 +175,8 @@ struct RegSolver {
       for (int r = 0; r < nl; r++) {
         const float dr = fbuf[r];
         if (dr == 0.0f) continue;
-        const float* Jr = Jl + r * 33;
-        const float a = dr * Jr[lane];
+        const float* Jr = Jl + r * JS;
+        const float a = lane < NVC ? dr * Jr[lane] : 0.0f;
 #pragma unroll
         for (int j = 0; j < NVC; j++) ch.h[j] = fmaf(a, Jr[j], ch.h[j]);
       }
@@ -373,7 +376,19 @@ MJH_DEV void solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int r
   }
   const int nst = nr < rj ? nr : rj;
   wave_fence();
-  for (int r = 0; r < nst; r++) s.Jl[r * 33 + lane] = dof ? s.Jg(r)[lane] : 0.0f;
+  {
+    // rows [0, nst) as one coalesced block; padding columns [nv, NVC) zero
+    const float* Jw = d.efc_J + (size_t)radr * nv;
+    for (int i = lane; i < nst * nv; i += 32) {
+      const int r = i / nv;
+      s.Jl[r * s.JS + i - r * nv] = Jw[i];
+    }
+    if (nv < NVC)
+      for (int i = lane; i < nst * (NVC - nv); i += 32) {
+        const int r = i / (NVC - nv);
+        s.Jl[r * s.JS + nv + i - r * (NVC - nv)] = 0.0f;
+      }
+  }
   if (m.opt.solver == mjhSOL_CG) {
     // first dof of the lane's kinematic-tree block of qLD (io.py:150-162)
     s.blk = lane;
@@ -390,7 +405,7 @@ MJH_DEV void solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int r
     if (r < nr) {
       const float arn = aref[k] / (float)nv;
       if (r < rj) {
-        const float* Jr = s.Jl + r * 33;
+        const float* Jr = s.Jl + r * s.JS;
 #pragma unroll
         for (int c = 0; c < NVC; c++)
           if (c < nv) a += Jr[c] * s.vbuf[c] - arn;
