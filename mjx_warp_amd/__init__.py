@@ -33,6 +33,8 @@ from .api import step_fused as step_fused
 from .api import StepGraph as StepGraph
 from .api import transmission as transmission
 from .api import xfrc_accumulate as xfrc_accumulate
+from .bridge import FunctionalStep as FunctionalStep
+from .bridge import Unroll as Unroll
 from .io import get_data_into as get_data_into
 from .tracing import EventTracer as EventTracer
 from .tracing import event_scope as event_scope
