@@ -381,6 +381,8 @@ class _Compiler:
 
   def _body_children(self, el, bid, childclass):
     for child in el:
+      if child.tag in _UNSUPPORTED_ELEMENTS:
+        raise NotImplementedError(f"MJCF element <{child.tag}> is not supported by this compiler")
       if child.tag == "geom":
         self._geom(child, bid, childclass)
       elif child.tag == "site":
@@ -1035,6 +1037,11 @@ def set_const(m: MjModel):
 _FIX_ATTR = re.compile(r'(=\s*"[^"]*")(?=[A-Za-z_])')
 
 
+# Elements that change the model's structure and that this compiler does
+# not expand: compiling without them would silently drop bodies.
+_UNSUPPORTED_ELEMENTS = ("replicate", "frame", "attach", "composite", "flexcomp", "include", "plugin")
+
+
 def from_xml_string(xml: str) -> MjModel:
   """Compiles an MJCF string (MjModel.from_xml_string counterpart)."""
   try:
@@ -1043,6 +1050,9 @@ def from_xml_string(xml: str) -> MjModel:
     # MJCF parsers accept attributes without separating whitespace
     # (e.g. type="ball"solreflimit=...); insert it and retry.
     root = ET.fromstring(_FIX_ATTR.sub(r"\1 ", xml))
+  for el in root.iter():
+    if el.tag in _UNSUPPORTED_ELEMENTS or (el.tag == "model" and "file" in el.attrib):
+      raise NotImplementedError(f"MJCF element <{el.tag}> is not supported by this compiler")
   if root.find("equality") is not None and len(root.find("equality")):
     m = _Compiler(root).compile()
     m.neq = len(root.find("equality"))
