@@ -297,7 +297,7 @@ int lds_solve_reg(const mjhModel& m) {
 Tune tune_tail_reg(const mjhModel& m) {
   Tune t = tune_for(m);
   const int nvc = (m.nv + 3) & ~3;
-  const int fit = (16384 / 2 - 64 - kRegRows - 3) / (nvc + 1);
+  const int fit = (16384 / 2 - lds_reg_floats(0, nvc) - 3) / (nvc + 1);
   t.rj = fit < kRegRows ? fit : kRegRows;
   return t;
 }
@@ -330,7 +330,7 @@ int launch_team(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, co
 // wavefront) when the register path applies, else 64-lane teams.
 template <class S>
 int launch(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, const char* name) {
-  if (reg_path(m)) return launch_team<S, kRegTeam>(m, d, s, (ldsw + 3) & ~3, name);
+  if (reg_path(m)) return launch_team<S, kRegTeam>(m, d, s, ((ldsw > kRegLdsMin ? ldsw : kRegLdsMin) + 3) & ~3, name);
   return launch_team<S, kTeam>(m, d, s, ldsw, name);
 }
 

@@ -21,6 +21,9 @@ namespace mjh {
 
 constexpr int kRegTeam = 32;
 constexpr int kRegNvMax = 32;
+// LDS floats a 32-lane team needs at least: RegChol's column broadcast
+// buffer (one float per lane)
+constexpr int kRegLdsMin = 32;
 
 MJH_DEV void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -80,7 +83,7 @@ template <int NVC>
 struct RegChol {
   static_assert(NVC % 4 == 0 && NVC <= kRegNvMax, "padded dof count");
   int lane;     // 0..31: the row this lane owns
-  float* cbuf;  // [32] LDS column broadcast buffer of the team
+  float* cbuf;  // [kRegLdsMin] LDS column broadcast buffer of the team
   float h[NVC];  // matrix row, then factor row (lower part)
   float rdiag;   // 1 / L[lane][lane]
 
