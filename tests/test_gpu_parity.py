@@ -279,6 +279,38 @@ def test_step_lying_humanoid_many_rows(fused):
   _close_or_enveloped("qacc", helpers.data_numpy(d, "qacc"), hd["qacc"], env["qacc"], 2e-3)
 
 
+@pytest.mark.parametrize("integrator", ["Euler", "implicitfast"])
+def test_step_affine_actuators(integrator):
+  """Velocity and position actuators (affine in qdot / q) on a three-hinge
+  arm: implicitfast folds their velocity derivatives into the integration
+  matrix (forward.py:245-405), Euler does not; one step on the GPU against
+  the oracle."""
+  from mjx_warp_amd import mjcf
+
+  mjm = mjcf.from_xml_string(f"""<mujoco><option timestep="0.005" integrator="{integrator}"/><worldbody>
+    <body pos="0 0 1"><joint name="a" type="hinge" axis="0 1 0" damping="0.2"/>
+      <geom type="capsule" fromto="0 0 0 0.3 0 0" size="0.04"/>
+      <body pos="0.3 0 0"><joint name="b" type="hinge" axis="0 1 0"/>
+        <geom type="capsule" fromto="0 0 0 0.3 0 0" size="0.04"/>
+        <body pos="0.3 0 0"><joint name="c" type="hinge" axis="1 0 0" damping="0.1"/>
+          <geom type="capsule" fromto="0 0 0 0.2 0 0" size="0.03"/></body></body></body>
+    </worldbody><actuator><velocity joint="a" kv="3"/><position joint="b" kp="20" kv="2"/>
+    <motor joint="c" gear="5"/></actuator></mujoco>""")
+  rng = np.random.default_rng(12)
+  nworld = 16
+  q = rng.normal(0, 0.5, (nworld, mjm.nq))
+  v = rng.normal(0, 1.0, (nworld, mjm.nv))
+  u = rng.uniform(-1, 1, (nworld, mjm.nu))
+  m, d, hm, hd = _gpu_and_oracle(mjm, nworld, q, v, u, nconmax_w=8, njmax_w=16)
+  assert int(m.actuator_affine_bias_gain) == 1
+  mjw.step(m, d)
+  torch.cuda.synchronize()
+  oracle.run("step", hm, hd)
+  _close("qvel", helpers.data_numpy(d, "qvel"), hd["qvel"], 2e-4)
+  _close("qpos", helpers.data_numpy(d, "qpos"), hd["qpos"], 2e-5)
+  _close("qacc", helpers.data_numpy(d, "qacc"), hd["qacc"], 2e-4)
+
+
 def _gpu_and_oracle(mjm, nworld, qpos, qvel=None, ctrl=None, nconmax_w=32, njmax_w=128):
   out = []
   for dev in ("cuda", "cpu"):

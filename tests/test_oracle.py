@@ -453,6 +453,55 @@ def test_damped_slide_is_backward_euler_in_the_damping(integrator):
   np.testing.assert_allclose(hd["qpos"][0], [0.25 + dt * v1], rtol=1e-12)
 
 
+@pytest.mark.parametrize("integrator", ["Euler", "implicitfast"])
+def test_velocity_actuator_is_implicit_in_implicitfast_only(integrator):
+  """A slide joint of mass m and damping b driven by a velocity actuator,
+  force kv (u - qdot): affine in qdot, so implicitfast moves kv into the
+  integration matrix (forward.py:245-405) and steps qdot' = qdot + dt (kv
+  (u - qdot) - b qdot) / (m + dt (kv + b)); Euler's implicit term holds the
+  damping only (denominator m + dt b)."""
+  xml = f"""<mujoco><option gravity="0 0 0" timestep="0.01" integrator="{integrator}"/><worldbody>
+    <body><joint name="j" type="slide" axis="1 0 0" damping="0.5"/><geom type="sphere" size="0.1" mass="2"/>
+    </body></worldbody><actuator><velocity joint="j" kv="4"/></actuator></mujoco>"""
+  mjm, m, d, hm, hd = helpers.load(xml)
+  assert int(m.actuator_affine_bias_gain) == 1
+  hd.arrays["qpos"][0] = [0.25]
+  hd.arrays["qvel"][0] = [0.7]
+  hd.arrays["ctrl"][0] = [1.5]
+  oracle.run("step", hm, hd)
+  dt, mass, b, kv, u, v0 = 0.01, 2.0, 0.5, 4.0, 1.5, 0.7
+  denom = mass + dt * (kv + b if integrator == "implicitfast" else b)
+  v1 = v0 + dt * (kv * (u - v0) - b * v0) / denom
+  np.testing.assert_allclose(hd["qvel"][0], [v1], rtol=1e-12)
+  np.testing.assert_allclose(hd["qpos"][0], [0.25 + dt * v1], rtol=1e-12)
+
+
+def test_resting_sphere_penetration_closed_form():
+  """A free sphere (mass m) settled on a plane with pyramidal condim-3
+  contact: at rest the four rows carry m g / 4 each, row force D aref with
+  aref = -k imp pos (constraint.py:22-73), D = imp / ((1 - imp) invweight),
+  invweight = 2 mu^2 (1 + mu^2) body_invweight0 / impratio and
+  body_invweight0 = 1 / m, k dmax^2 = 1 / timeconst^2 (dampratio 1), and imp
+  = dmax beyond the impedance width.  So the depth is g (1 - dmax)
+  timeconst^2 -- independent of m, which the test varies."""
+  depths = []
+  for mass in (0.5, 2.0):
+    xml = f"""<mujoco><option timestep="0.002" iterations="100" tolerance="1e-12"/><worldbody>
+      <geom type="plane" size="5 5 .1" solimp="0.9 0.95 0.00001 0.5 2"/>
+      <body pos="0 0 0.1"><freejoint/><geom type="sphere" size="0.1" mass="{mass}"
+        solimp="0.9 0.95 0.00001 0.5 2"/></body></worldbody></mujoco>"""
+    mjm, m, d, hm, hd = helpers.load(xml, nconmax=8, njmax=16)
+    np.testing.assert_allclose(np.asarray(hm.arrays["body_invweight0"])[1, 0], 1.0 / mass, rtol=1e-9)
+    for _ in range(600):
+      oracle.run("step", hm, hd)
+    # settled vertically (the pyramid's friction rows leave a slow ~1e-8
+    # rad/s spin, which does not load the normal direction)
+    assert abs(hd["qvel"][0][2]) < 1e-9 and np.max(np.abs(hd["qvel"][0])) < 1e-6
+    depths.append(0.1 - hd["qpos"][0][2])
+  expected = 9.81 * (1.0 - 0.95) * 0.02 ** 2
+  np.testing.assert_allclose(depths, [expected, expected], rtol=1e-6)
+
+
 # ---------------------------------------------------------------------------
 # primitive contacts: distance, position midway between the surfaces, normal
 def _contacts(xml, qpos=None):
