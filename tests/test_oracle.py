@@ -502,6 +502,57 @@ def test_resting_sphere_penetration_closed_form():
   np.testing.assert_allclose(depths, [expected, expected], rtol=1e-6)
 
 
+def test_resting_sphere_penetration_on_the_impedance_curve():
+  """As above with the impedance width (0.01) above the resting depth: imp
+  follows MuJoCo's solimp sigmoid, imp(x) = dmin + y(x) (dmax - dmin),
+  x = depth / width, y = x^p / mid^(p-1) below mid, else 1 - (1-x)^p /
+  (1-mid)^(p-1); the steady depth is the root of depth = g (1 - imp) /
+  (k imp^2) with k = 1 / (dmax^2 timeconst^2)."""
+  dmin, dmax, width, mid, power, tc, g = 0.9, 0.95, 0.01, 0.3, 3.0, 0.02, 9.81
+  solimp = f"{dmin} {dmax} {width} {mid} {power}"
+  xml = f"""<mujoco><option timestep="0.002" iterations="100" tolerance="1e-12"/><worldbody>
+    <geom type="plane" size="5 5 .1" solimp="{solimp}"/>
+    <body pos="0 0 0.1"><freejoint/><geom type="sphere" size="0.1" mass="1" solimp="{solimp}"/></body>
+    </worldbody></mujoco>"""
+  mjm, m, d, hm, hd = helpers.load(xml, nconmax=8, njmax=16)
+  for _ in range(800):
+    oracle.run("step", hm, hd)
+  assert abs(hd["qvel"][0][2]) < 1e-9
+  depth = 0.1 - hd["qpos"][0][2]
+
+  def imp(x):
+    y = x ** power / mid ** (power - 1) if x < mid else 1.0 - (1.0 - x) ** power / (1.0 - mid) ** (power - 1)
+    return dmin + y * (dmax - dmin)
+
+  k = 1.0 / (dmax * dmax * tc * tc)
+  expected = scipy.optimize.brentq(lambda z: z - g * (1.0 - imp(z / width)) / (k * imp(z / width) ** 2),
+                                   1e-9, width)
+  assert expected < width and imp(expected / width) < dmax - 1e-3  # on the curve, not at dmax
+  np.testing.assert_allclose(depth, expected, rtol=1e-6)
+
+
+def test_set_const_closed_forms():
+  """mj_setConst quantities of a free sphere (mass m, inertia I = 2/5 m r^2
+  about every axis) and of a hinge pendulum: meaninertia is the mean of
+  diag(M) = (3m + 3I) / 6; dof_invweight0 is 1/m on the translational and
+  1/I on the rotational dofs; body_invweight0 = (1/m, 1/I); the pendulum's
+  hinge has dof_invweight0 = 1 / (I_axis + m l^2)."""
+  mass, r = 2.5, 0.2
+  inertia = 0.4 * mass * r * r
+  mjm = mjcf.from_xml_string(f"""<mujoco><worldbody><body><freejoint/>
+    <geom type="sphere" size="{r}" mass="{mass}"/></body></worldbody></mujoco>""")
+  np.testing.assert_allclose(mjm.stat.meaninertia, (3 * mass + 3 * inertia) / 6, rtol=1e-9)
+  np.testing.assert_allclose(mjm.dof_invweight0, [1 / mass] * 3 + [1 / inertia] * 3, rtol=1e-9)
+  np.testing.assert_allclose(mjm.body_invweight0[1], [1 / mass, 1 / inertia], rtol=1e-9)
+  # pendulum: a point-like sphere at distance l from a hinge about y
+  l, ms, rs = 0.7, 1.5, 0.05
+  mjm = mjcf.from_xml_string(f"""<mujoco><worldbody><body><joint type="hinge" axis="0 1 0"/>
+    <geom type="sphere" pos="{l} 0 0" size="{rs}" mass="{ms}"/></body></worldbody></mujoco>""")
+  iaxis = 0.4 * ms * rs * rs + ms * l * l
+  np.testing.assert_allclose(mjm.dof_invweight0, [1 / iaxis], rtol=1e-9)
+  np.testing.assert_allclose(mjm.stat.meaninertia, iaxis, rtol=1e-9)
+
+
 # ---------------------------------------------------------------------------
 # primitive contacts: distance, position midway between the surfaces, normal
 def _contacts(xml, qpos=None):
