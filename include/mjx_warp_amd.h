@@ -210,7 +210,7 @@ typedef struct mjhOption {
 #define MJH_NXN_TABLE_MAX 100
 
 /* ---- data ---- */
-#define MJH_DATA_SIZES(X) X(nworld) X(nconmax) X(njmax) X(qM_rows) X(qM_cols)
+#define MJH_DATA_SIZES(X) X(nworld) X(nconmax) X(njmax) X(qM_rows) X(qM_cols) X(qMd_n)
 
 #define MJH_DATA_ARRAYS(X) \
   X(I, ncon, (1)) \
@@ -247,6 +247,8 @@ typedef struct mjhOption {
   X(R, qM, (nworld, qM_rows, qM_cols)) \
   X(R, qLD, (nworld, qM_rows, qM_cols)) \
   X(R, qLDiagInv, (nworld, nv)) \
+  X(R, qMd, (nworld, qMd_n, qMd_n)) \
+  X(R, qLDd, (nworld, qMd_n, qMd_n)) \
   X(R, actuator_velocity, (nworld, nu)) \
   X(R, cvel, (nworld, nbody, 6)) \
   X(R, cdof_dot, (nworld, nv, 6)) \
@@ -351,6 +353,13 @@ typedef struct mjhOption {
  * counts pairs / contacts / rows dropped because a pool was full (the
  * reference drops contacts silently and writes rows out of bounds,
  * collision_primitive.py:64-65, constraint.py:119,162).
+ * qM / qLD have the reference's layout: dense [nworld, nv, nv] (qM_rows =
+ * qM_cols = nv; qLD the Cholesky factor L, M = L L') or, for sparse models,
+ * MuJoCo's [nworld, 1, nM] (qM_rows = 1, qM_cols = nM; qLD the L'DL factor
+ * with qLDiagInv = 1 / D).  qMd / qLDd are the dense matrix and Cholesky
+ * factor every kernel works with: for dense models qMd_n = 0 and the caller
+ * passes qM's and qLD's pointers as qMd and qLDd; for sparse models
+ * qMd_n = nv and they are separate arrays.
  * Size aliases: nworld2 = 2*nworld, nworld1 = nworld+1,
  * nworld_ngeom = nworld*ngeom. */
 

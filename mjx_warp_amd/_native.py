@@ -93,7 +93,7 @@ def model_sizes(m) -> dict:
 def data_sizes(m, d) -> dict:
   s = model_sizes(m)
   s.update(nworld=int(d.nworld), nconmax=int(d.nconmax), njmax=int(d.njmax), qM_rows=int(d.qM_rows),
-           qM_cols=int(d.qM_cols))
+           qM_cols=int(d.qM_cols), qMd_n=int(d.qMd_n))
   s.update(nworld2=2 * s["nworld"], nworld1=s["nworld"] + 1, nworld_ngeom=s["nworld"] * s["ngeom"])
   return s
 
@@ -160,11 +160,14 @@ def data_struct(m, d):
   _, arrays = layout()["data"]
   sizes = data_sizes(m, d)
   s = DataStruct()
-  for k in ("nworld", "nconmax", "njmax", "qM_rows", "qM_cols"):
+  for k in ("nworld", "nconmax", "njmax", "qM_rows", "qM_cols", "qMd_n"):
     setattr(s, k, sizes[k])
   for kind, name, dims in arrays:
     node, attr = data_node(d, name)
     setattr(s, name, _ptr(getattr(node, attr), name, kind, shape_of(dims, sizes)))
+  if sizes["qMd_n"] == 0:
+    # dense model: the kernels' dense working matrices are qM and qLD
+    s.qMd, s.qLDd = s.qM, s.qLD
   object.__setattr__(d, "_abi", s)
   object.__setattr__(d, "_abi_model", m)
   return s

@@ -216,14 +216,14 @@ __global__ void __launch_bounds__(64) k_mul_m(mjhModel m, mjhData d, float* res,
   if (!t.valid) return;
   if (skip && skip[t.world]) return;
   const size_t o = (size_t)t.world * m.nv;
-  mul_m(m, d.qM + o * m.nv, res + o, vec + o, t);
+  mul_m(m, d.qMd + o * m.nv, res + o, vec + o, t);
 }
 __global__ void __launch_bounds__(64) k_solve_m(mjhModel m, mjhData d, float* x, const float* y, int ldsw) {
   extern __shared__ float smem[];
   Team<kTeam> t(d.nworld);
   if (!t.valid) return;
   const size_t o = (size_t)t.world * m.nv;
-  solve_blocks(m, d.qLD + o * m.nv, x + o, y + o, smem + t.slot * ldsw, t);
+  solve_blocks(m, d.qLDd + o * m.nv, x + o, y + o, smem + t.slot * ldsw, t);
 }
 __global__ void __launch_bounds__(64) k_xfrc(mjhModel m, mjhData d, float* qfrc) {
   Team<kTeam> t(d.nworld);

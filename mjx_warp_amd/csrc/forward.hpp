@@ -156,7 +156,7 @@ MJH_DEV void fwd_acceleration_force(const mjhModel& m, const mjhData& d, const T
 template <int TEAM>
 MJH_DEV void fwd_acceleration_solve(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int w = t.world, nv = m.nv;
-  if (t.valid) solve_blocks(m, WPTR(d.qLD, w, nv * nv), WPTR(d.qacc_smooth, w, nv), WPTR(d.qfrc_smooth, w, nv), lds, t);
+  if (t.valid) solve_blocks(m, WPTR(d.qLDd, w, nv * nv), WPTR(d.qacc_smooth, w, nv), WPTR(d.qfrc_smooth, w, nv), lds, t);
   sync();
 }
 
@@ -249,7 +249,7 @@ MJH_DEV void euler(const mjhModel& m, const mjhData& d, float* lds, const Team<T
       const float b = dof ? d.qfrc_smooth[w * nv + lane] + d.qfrc_constraint[w * nv + lane] : 0.0f;
       const float x = with_nvc(nv, [&](auto nvc) {
         RegChol<decltype(nvc)::value> c{lane, lds};
-        c.load_sym(WPTR(d.qM, w, nv * nv), nv, beg, end, damp);
+        c.load_sym(WPTR(d.qMd, w, nv * nv), nv, beg, end, damp);
         c.factor();
         return c.solve(b);
       });
@@ -261,7 +261,7 @@ MJH_DEV void euler(const mjhModel& m, const mjhData& d, float* lds, const Team<T
     return;
   }
   if (t.valid) {
-    const float* qM = WPTR(d.qM, w, nv * nv);
+    const float* qM = WPTR(d.qMd, w, nv * nv);
     const float* qs = WPTR(d.qfrc_smooth, w, nv);
     const float* qc = WPTR(d.qfrc_constraint, w, nv);
     float* qacc_int = WPTR(d.qacc_integration, w, nv);
@@ -330,7 +330,7 @@ MJH_DEV void implicit(const mjhModel& m, const mjhData& d, float* lds, const Tea
       const float x = with_nvc(nv, [&](auto nvc) {
         constexpr int NVC = decltype(nvc)::value;
         RegChol<NVC> c{lane, lds};
-        c.load_sym(WPTR(d.qM, w, nv * nv), nv, beg, end, damp);
+        c.load_sym(WPTR(d.qMd, w, nv * nv), nv, beg, end, damp);
         if (act_on) {
           for (int a = 0; a < nu; a++) {
             const float* mr = moment + a * nv;
@@ -357,7 +357,7 @@ MJH_DEV void implicit(const mjhModel& m, const mjhData& d, float* lds, const Tea
     return;
   }
   if (t.valid) {
-    const float* qM = WPTR(d.qM, w, nv * nv);
+    const float* qM = WPTR(d.qMd, w, nv * nv);
     const float* moment = WPTR(d.actuator_moment, w, nu * nv);
     float* qMi = WPTR(d.qM_integration, w, nv * nv);
     float* qfi = WPTR(d.qfrc_integration, w, nv);

@@ -276,7 +276,7 @@ MJH_DEV void crb(const mjhModel& m, const mjhData& d, float* lds, const Team<TEA
     }
   }
   sync();
-  float* qM = WPTR(d.qM, w, nv * nv);
+  float* qM = WPTR(d.qMd, w, nv * nv);
   if (t.valid)
     for (int i = t.lane; i < nv * nv; i += TEAM) qM[i] = lqM[i];
   sync();
@@ -377,7 +377,7 @@ template <int TEAM>
 MJH_DEV void factor_m(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int nv = m.nv;
   if (!t.valid) return;  // padding slot; a workgroup is one wavefront
-  factor_blocks(m, WPTR(d.qM, t.world, nv * nv), WPTR(d.qLD, t.world, nv * nv), lds, t);
+  factor_blocks(m, WPTR(d.qMd, t.world, nv * nv), WPTR(d.qLDd, t.world, nv * nv), lds, t);
 }
 
 // ---------------------------------------------------------------------------

@@ -86,7 +86,7 @@ struct SolverWorld {
     if (t.lane == 0) d.efc_grad_dot[w] = gd;
     sync();
     if (m.opt.solver == mjhSOL_CG) {
-      solve_blocks(m, d.qLD + (size_t)w * nv * nv, Mgrad, grad, lds, t);
+      solve_blocks(m, d.qLDd + (size_t)w * nv * nv, Mgrad, grad, lds, t);
     } else {
       float* H = lds;
       float* x = lds + nv * nv;
@@ -341,7 +341,7 @@ MJH_DEV void solve(const mjhModel& m, const mjhData& d, float* lds, const Team<T
                       d.qacc + (size_t)w * nv, d.efc_Ma + (size_t)w * nv, d.efc_grad + (size_t)w * nv,
                       d.efc_Mgrad + (size_t)w * nv, d.efc_search + (size_t)w * nv, d.efc_mv + (size_t)w * nv,
                       d.qfrc_smooth + (size_t)w * nv, d.qacc_smooth + (size_t)w * nv,
-                      d.qfrc_constraint + (size_t)w * nv, d.qM + (size_t)w * nv * nv, lds};
+                      d.qfrc_constraint + (size_t)w * nv, d.qMd + (size_t)w * nv * nv, lds};
   const float* ws = d.qacc_warmstart + (size_t)w * nv;
   for (int i = t.lane; i < nv; i += TEAM) s.qacc[i] = ws[i];
   sync();

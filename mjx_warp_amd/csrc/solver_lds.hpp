@@ -285,7 +285,7 @@ MJH_DEV bool solve_lds(const mjhModel& m, const mjhData& d, float* lds, int rmax
   if (nr > rmax) return false;
   LdsSolver<TEAM> s{m, d, t, w, nv, radr, nr};
   float* p = lds;
-  s.M = d.qM + (size_t)w * nv * nv;
+  s.M = d.qMd + (size_t)w * nv * nv;
   s.H = p; p += nv * nv;
   s.qacc = p; p += nv;
   s.Ma = p; p += nv;
@@ -318,7 +318,7 @@ MJH_DEV bool solve_lds(const mjhModel& m, const mjhData& d, float* lds, int rmax
   // stage the world's inputs
   const size_t o = (size_t)w * nv;
   if (m.opt.solver == mjhSOL_CG) {
-    const float* qLD = d.qLD + o * nv;
+    const float* qLD = d.qLDd + o * nv;
     for (int k = t.lane; k < nv * nv; k += TEAM) s.H[k] = qLD[k];
   }
   for (int i = t.lane; i < nv; i += TEAM) {

@@ -192,7 +192,7 @@ struct RegSolver {
       ch.factor();
     } else if (!fact_valid) {
       // CG preconditions with the factor of M, fixed during the solve
-      ch.load_lower(d.qLD + (size_t)w * nv * nv, nv, blk);
+      ch.load_lower(d.qLDd + (size_t)w * nv * nv, nv, blk);
       fact_valid = true;
     }
     Mgrad = ch.solve(grad);
@@ -365,7 +365,7 @@ MJH_DEV void solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int r
   s.qacc = dof ? d.qacc_warmstart[o + lane] : 0.0f;
   s.qs = dof ? d.qfrc_smooth[o + lane] : 0.0f;
   s.qas = dof ? d.qacc_smooth[o + lane] : 0.0f;
-  s.M = d.qM + o * nv;
+  s.M = d.qMd + o * nv;
   float aref[RS];
 #pragma unroll
   for (int k = 0; k < RS; k++) {

@@ -348,7 +348,7 @@ void crb(const Model<T>& m, const Data<T>& d, int w) {
     if (p == 0) continue;
     for (int k = 0; k < 10; k++) c[10 * p + k] += c[10 * b + k];
   }
-  T* M = d.qM + (size_t)w * nv * nv;
+  T* M = d.qMd + (size_t)w * nv * nv;
   std::fill(M, M + nv * nv, T(0));
   const T* cdof = d.cdof + (size_t)w * nv * 6;
   for (int i = 0; i < nv; i++) {
@@ -413,8 +413,8 @@ void each_block(const Model<T>& m, F f) {
 template <class T>
 void factor_m(const Model<T>& m, const Data<T>& d, int w) {
   const int nv = m.nv;
-  const T* M = d.qM + (size_t)w * nv * nv;
-  T* L = d.qLD + (size_t)w * nv * nv;
+  const T* M = d.qMd + (size_t)w * nv * nv;
+  T* L = d.qLDd + (size_t)w * nv * nv;
   each_block(m, [&](int off, int n) { cholesky_block(M, L, off, n, nv); });
 }
 
@@ -422,7 +422,7 @@ void factor_m(const Model<T>& m, const Data<T>& d, int w) {
 template <class T>
 void solve_m(const Model<T>& m, const Data<T>& d, int w, T* x, const T* y) {
   const int nv = m.nv;
-  const T* L = d.qLD + (size_t)w * nv * nv;
+  const T* L = d.qLDd + (size_t)w * nv * nv;
   each_block(m, [&](int off, int n) { cho_solve_block(L, off, n, nv, x + (size_t)w * nv, y + (size_t)w * nv); });
 }
 
@@ -1368,7 +1368,7 @@ void solve(const Model<T>& m, const Data<T>& d, int w) {
   const size_t o = (size_t)w * nv;
   Solver<T> s{m, d, w, nv, d.world_efc[2 * w], d.world_efc[2 * w + 1], d.qacc + o, d.efc_Ma + o, d.efc_grad + o,
               d.efc_Mgrad + o, d.efc_search + o, d.efc_mv + o, d.qfrc_constraint + o, d.qfrc_smooth + o,
-              d.qacc_smooth + o, d.qM + o * nv};
+              d.qacc_smooth + o, d.qMd + o * nv};
   for (int i = 0; i < nv; i++) s.qacc[i] = d.qacc_warmstart[o + i];
   d.efc_cost[w] = std::numeric_limits<T>::infinity();
   d.efc_solver_niter[w] = 0;
@@ -1462,7 +1462,7 @@ void euler(const Model<T>& m, const Data<T>& d, int w) {
   const int nv = m.nv;
   const size_t o = (size_t)w * nv;
   std::vector<T> A(nv * nv), L(nv * nv, T(0)), f(nv);
-  for (int k = 0; k < nv * nv; k++) A[k] = d.qM[o * nv + k];
+  for (int k = 0; k < nv * nv; k++) A[k] = d.qMd[o * nv + k];
   for (int i = 0; i < nv; i++) {
     A[i * nv + i] += m.dof_damping[i] * m.opt.timestep;
     f[i] = d.qfrc_smooth[o + i] + d.qfrc_constraint[o + i];
@@ -1495,7 +1495,7 @@ void implicit(const Model<T>& m, const Data<T>& d, int w) {
       avi[a] = bv + gv * c;
     }
   T* Mi = d.qM_integration + o * nv;
-  const T* M = d.qM + o * nv;
+  const T* M = d.qMd + o * nv;
   const T* mom = d.actuator_moment + (size_t)w * nu * nv;
   each_act_tile(m, m.nqLD_tile, [&](int onv, int onu, int tnv, int tnu) {
     if (tnv == 0) return;

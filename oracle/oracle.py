@@ -129,10 +129,13 @@ class HostData:
   def struct(self):
     _, Data = structs(self.real)
     s = Data()
-    for k in ("nworld", "nconmax", "njmax", "qM_rows", "qM_cols"):
+    for k in ("nworld", "nconmax", "njmax", "qM_rows", "qM_cols", "qMd_n"):
       setattr(s, k, int(self.sizes[k]))
     for name, a in self.arrays.items():
       setattr(s, name, a.ctypes.data if a.size else None)
+    if int(self.sizes["qMd_n"]) == 0:
+      # dense model: the dense working matrices are qM and qLD themselves
+      s.qMd, s.qLDd = s.qM, s.qLD
     return s
 
 

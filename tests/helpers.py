@@ -25,7 +25,8 @@ def host_model(m, real=np.float64) -> oracle.HostModel:
 
 def host_data(m, d, real=np.float64) -> oracle.HostData:
   _, darrays = _native.layout()["data"]
-  sizes = dict(nworld=d.nworld, nconmax=d.nconmax, njmax=d.njmax, qM_rows=d.qM_rows, qM_cols=d.qM_cols)
+  sizes = dict(nworld=d.nworld, nconmax=d.nconmax, njmax=d.njmax, qM_rows=d.qM_rows, qM_cols=d.qM_cols,
+               qMd_n=d.qMd_n)
   arrays = {}
   for _, name, _ in darrays:
     node, attr = _native.data_node(d, name)
