@@ -384,7 +384,15 @@ int do_forward(const mjhModel& m, const mjhData& d, hipStream_t s) {
   return do_solve(m, d, s);
 }
 
+// forward.py:268: implicitfast needs the dense mass matrix
+int check_implicit_dense(const mjhModel& m) {
+  if (m.opt.integrator == mjhINT_IMPLICITFAST && m.opt.is_sparse)
+    return fail(MJH_ERR_UNSUPPORTED, "implicit integration requires a dense mass matrix");
+  return MJH_OK;
+}
+
 int do_integrate(const mjhModel& m, const mjhData& d, hipStream_t s) {
+  TRY(check_implicit_dense(m));
   if (m.opt.integrator == mjhINT_EULER) return launch<StEuler>(m, d, s, lds_linalg(m), "euler");
   if (m.opt.integrator == mjhINT_IMPLICITFAST) return launch<StImplicit>(m, d, s, lds_linalg(m), "implicit");
   return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
@@ -393,6 +401,7 @@ int do_integrate(const mjhModel& m, const mjhData& d, hipStream_t s) {
 int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
   if (m.opt.integrator != mjhINT_EULER && m.opt.integrator != mjhINT_IMPLICITFAST)
     return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
+  TRY(check_implicit_dense(m));
   TRY(reset(d, s, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3));
   if (reg_path(m)) {
     TRY((launch_team<StFusedForward, kRegTeam>(m, d, s, lds_forward(m), "step_fused_forward")));
@@ -415,13 +424,19 @@ const char* mjh_last_error(void) { return g_last_error.c_str(); }
 
 int mjh_check(const mjhModel* m, const mjhData* d) {
   if (!m || !d) return fail(MJH_ERR_INVALID, "null model or data");
-  if (m->opt.is_sparse) return fail(MJH_ERR_UNSUPPORTED, "sparse mass matrix is not implemented");
   if (m->opt.cone != mjhCONE_PYRAMIDAL) return fail(MJH_ERR_UNSUPPORTED, "only the pyramidal cone is implemented");
   if (m->opt.solver != mjhSOL_CG && m->opt.solver != mjhSOL_NEWTON)
     return fail(MJH_ERR_UNSUPPORTED, "Solver: " + std::to_string(m->opt.solver) + " is unsupported.");
   if (m->nlevel < 1 || m->nbody < 1) return fail(MJH_ERR_INVALID, "model has no bodies");
   if (d->nworld < 0 || d->nconmax < 0 || d->njmax < 0) return fail(MJH_ERR_INVALID, "negative sizes");
-  if (d->qM_rows != m->nv || d->qM_cols != m->nv) return fail(MJH_ERR_INVALID, "qM must be dense [nworld, nv, nv]");
+  if (m->opt.is_sparse) {
+    if (d->qM_rows != 1 || d->qM_cols != m->nM || d->qMd_n != m->nv)
+      return fail(MJH_ERR_INVALID, "sparse model: qM must be [nworld, 1, nM] and qMd [nworld, nv, nv]");
+  } else if (d->qM_rows != m->nv || d->qM_cols != m->nv) {
+    return fail(MJH_ERR_INVALID, "qM must be dense [nworld, nv, nv]");
+  }
+  if (m->nv > 0 && d->nworld > 0 && (!d->qMd || !d->qLDd))
+    return fail(MJH_ERR_INVALID, "qMd / qLDd must be set (qM / qLD for a dense model)");
   return MJH_OK;
 }
 
@@ -472,6 +487,7 @@ int mjh_euler(const mjhModel* m, const mjhData* d, mjhStream stream) {
   return launch<StEuler>(*m, *d, (hipStream_t)stream, lds_linalg(*m), "euler");
 }
 int mjh_implicit(const mjhModel* m, const mjhData* d, mjhStream stream) {
+  if (m->opt.is_sparse) return fail(MJH_ERR_UNSUPPORTED, "implicit integration requires a dense mass matrix");
   return launch<StImplicit>(*m, *d, (hipStream_t)stream, lds_linalg(*m), "implicit");
 }
 int mjh_forward(const mjhModel* m, const mjhData* d, mjhStream stream) {
@@ -480,6 +496,7 @@ int mjh_forward(const mjhModel* m, const mjhData* d, mjhStream stream) {
 int mjh_step(const mjhModel* m, const mjhData* d, mjhStream stream) {
   if (m->opt.integrator != mjhINT_EULER && m->opt.integrator != mjhINT_IMPLICITFAST)
     return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m->opt.integrator) + " not implemented.");
+  TRY(check_implicit_dense(*m));
   TRY(do_forward(*m, *d, (hipStream_t)stream));
   return do_integrate(*m, *d, (hipStream_t)stream);
 }

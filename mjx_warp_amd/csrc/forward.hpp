@@ -229,6 +229,27 @@ MJH_DEV void advance(const mjhModel& m, const mjhData& d, const float* qacc_all,
 // forward.py:165-242 euler (dense eulerdamp: (M + dt diag(damping)) qacc =
 // qfrc_smooth + qfrc_constraint, factor and solve per tree block);
 // lds: nv*nv + nv floats
+// forward.py:171-193 eulerdamp_sparse, for sparse models: qM_integration =
+// qM + dt diag(damping) in the sparse layout, its L'DL factor and
+// qfrc_integration (qacc_integration itself comes from the dense solve)
+template <int TEAM>
+MJH_DEV void euler_sparse_outputs(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
+  const int w = t.world, nv = m.nv, nM = m.nM;
+  float* Mi = WPTR(d.qM_integration, w, nM);
+  if (t.valid) {
+    const float* S = WPTR(d.qM, w, nM);
+    for (int k = t.lane; k < nM; k += TEAM) Mi[k] = S[k];
+  }
+  sync();
+  if (t.valid)
+    for (int i = t.lane; i < nv; i += TEAM) {
+      Mi[m.dof_Madr[i]] += m.opt.timestep * m.dof_damping[i];
+      d.qfrc_integration[w * nv + i] = d.qfrc_smooth[w * nv + i] + d.qfrc_constraint[w * nv + i];
+    }
+  sync();
+  factor_sparse(m, Mi, WPTR(d.qLD_integration, w, nM), WPTR(d.qLDiagInv_integration, w, nv), lds, t);
+}
+
 template <int TEAM>
 MJH_DEV void euler(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int w = t.world, nv = m.nv;
@@ -237,6 +258,7 @@ MJH_DEV void euler(const mjhModel& m, const mjhData& d, float* lds, const Team<T
     sync();
     return;
   }
+  if (m.opt.is_sparse) euler_sparse_outputs(m, d, lds, t);
   if constexpr (TEAM == kRegTeam) {
     // (M + dt diag(damping)) qacc_int = qfrc_smooth + qfrc_constraint, the
     // matrix row and its factor in registers

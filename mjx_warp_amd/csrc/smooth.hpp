@@ -232,8 +232,51 @@ MJH_DEV void com_pos(const mjhModel& m, const mjhData& d, float* lds, const Team
 
 // ---------------------------------------------------------------------------
 // smooth.py:248-314 crb (dense qM; the sparse layout is not built)
+// MuJoCo's sparse layout of the dense nv x nv matrix A into S (smooth.py:
+// 281-294 qM_sparse): row i holds A(i, j) for j = i, parent(i), ... from
+// dof_Madr[i]; lanes over rows
+template <int TEAM>
+MJH_DEV void dense_to_sparse(const mjhModel& m, const float* A, float* S, const Team<TEAM>& t) {
+  for (int i = t.lane; i < m.nv; i += TEAM) {
+    int a = m.dof_Madr[i];
+    for (int j = i; j >= 0; j = m.dof_parentid[j]) S[a++] = A[i * m.nv + j];
+  }
+}
+
+// smooth.py:317-350 _factor_i_sparse: the L'DL factor of the sparse matrix S
+// (global) into L (unit L below the diagonal, D on it) and Dinv = 1 / D,
+// leaves first as MuJoCo; worked in LDS (nM floats), lanes over the entries
+// of each updated row.  Every world runs the same loop (model-only bounds).
+template <int TEAM>
+MJH_DEV void factor_sparse(const mjhModel& m, const float* S, float* L, float* Dinv, float* lds,
+                           const Team<TEAM>& t) {
+  const int nv = m.nv, nM = m.nM;
+  if (t.valid)
+    for (int k = t.lane; k < nM; k += TEAM) lds[k] = S[k];
+  sync();
+  for (int k = nv - 1; k >= 0; k--) {
+    const int ak = m.dof_Madr[k];
+    int p = 1;
+    for (int i = m.dof_parentid[k]; i >= 0; i = m.dof_parentid[i], p++) {
+      const float tmp = lds[ak + p] / lds[ak];
+      const int ai = m.dof_Madr[i];
+      const int len = m.dof_Madr[i + 1] - ai;  // i < k <= nv - 1
+      for (int q = t.lane; q < len; q += TEAM) lds[ai + q] -= lds[ak + p + q] * tmp;
+      sync();
+      if (t.lane == 0) lds[ak + p] = tmp;
+      sync();
+    }
+  }
+  if (t.valid) {
+    for (int k = t.lane; k < nM; k += TEAM) L[k] = lds[k];
+    for (int k = t.lane; k < nv; k += TEAM) Dinv[k] = 1.0f / lds[m.dof_Madr[k]];
+  }
+  sync();
+}
+
 // lds: lds_crb_floats; holds crb, cdof, dof_parentid and the mass matrix,
-// which is written to qM row by row at the end
+// which is written to qMd row by row at the end (and, for sparse models, to
+// qM in MuJoCo's sparse layout)
 template <int TEAM>
 MJH_DEV void crb(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int w = t.world;
@@ -277,8 +320,10 @@ MJH_DEV void crb(const mjhModel& m, const mjhData& d, float* lds, const Team<TEA
   }
   sync();
   float* qM = WPTR(d.qMd, w, nv * nv);
-  if (t.valid)
+  if (t.valid) {
     for (int i = t.lane; i < nv * nv; i += TEAM) qM[i] = lqM[i];
+    if (m.opt.is_sparse) dense_to_sparse(m, lqM, WPTR(d.qM, w, m.nM), t);
+  }
   sync();
 }
 
@@ -372,12 +417,17 @@ MJH_DEV void solve_blocks(const mjhModel& m, const float* L, float* x, const flo
   });
 }
 
-// smooth.py:392-395 factor_m (dense)
+// smooth.py:383-395 factor_m: the dense Cholesky factor qLDd of qMd, and
+// for sparse models MuJoCo's L'DL factor of qM into qLD / qLDiagInv.
+// lds: nv * nv floats (at least nM).
 template <int TEAM>
 MJH_DEV void factor_m(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
-  const int nv = m.nv;
-  if (!t.valid) return;  // padding slot; a workgroup is one wavefront
-  factor_blocks(m, WPTR(d.qMd, t.world, nv * nv), WPTR(d.qLDd, t.world, nv * nv), lds, t);
+  const int nv = m.nv, w = t.world;
+  if (t.valid) factor_blocks(m, WPTR(d.qMd, w, nv * nv), WPTR(d.qLDd, w, nv * nv), lds, t);
+  if (m.opt.is_sparse) {
+    sync();
+    factor_sparse(m, WPTR(d.qM, w, m.nM), WPTR(d.qLD, w, m.nM), WPTR(d.qLDiagInv, w, nv), lds, t);
+  }
 }
 
 // ---------------------------------------------------------------------------

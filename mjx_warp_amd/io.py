@@ -110,22 +110,23 @@ def model_tables(mjm) -> dict:
     if upd:
       t["qLD_update_tree"] = np.array([u for k in range(len(upd)) for u in upd[k]], np.int32).reshape(-1, 3)
       t["qLD_update_treeadr"] = np.cumsum([0] + [len(upd[k]) for k in range(len(upd))])[:-1].astype(np.int32)
-  else:
-    # one dense tile per kinematic tree, grouped by tile size (io.py:150-162)
-    tiles = {}
-    for n, beg in enumerate(roots):
-      end = nv if n == len(roots) - 1 else roots[n + 1]
-      tiles.setdefault(end - beg, []).append(beg)
-    keys = sorted(tiles)
-    if keys:
-      t["qLD_tile"] = np.concatenate([tiles[k] for k in keys]).astype(np.int32)
-      t["qLD_tileadr"] = np.cumsum([0] + [len(tiles[k]) for k in keys])[:-1].astype(np.int32)
-      t["qLD_tilesize"] = np.array(keys, np.int32)
+  # one dense tile per kinematic tree, grouped by tile size (io.py:150-162).
+  # The reference builds these for dense models only; here the kernels work
+  # on the dense matrix for sparse models too, so every model has them.
+  tiles = {}
+  for n, beg in enumerate(roots):
+    end = nv if n == len(roots) - 1 else roots[n + 1]
+    tiles.setdefault(end - beg, []).append(beg)
+  keys = sorted(tiles)
+  if keys:
+    t["qLD_tile"] = np.concatenate([tiles[k] for k in keys]).astype(np.int32)
+    t["qLD_tileadr"] = np.cumsum([0] + [len(tiles[k]) for k in keys])[:-1].astype(np.int32)
+    t["qLD_tilesize"] = np.array(keys, np.int32)
 
   # actuator_moment tiles: (tree dofs, tree actuators) (io.py:164-204)
   for k in ("offset_nv", "offset_nu", "tileadr", "tilesize_nv", "tilesize_nu"):
     t["actuator_moment_" + k] = np.zeros(0, np.int32)
-  if not sparse and roots:
+  if roots:  # (as the dense tiles above: every model)
     tree_id = mjm.dof_treeid[roots]
     num_trees = int(np.max(tree_id))
     act_tree = mjm.body_treeid[mjm.jnt_bodyid[mjm.actuator_trnid[:, 0]]] if mjm.nu else np.zeros(0)
@@ -258,8 +259,6 @@ def put_model(mjm, device=None) -> types.Model:
   host = model_arrays(mjm, _native.model_sizes(m), tables)
   for kind, name, _ in arrays:
     setattr(m, name, _tensor(host[name], kind, device))
-  if m.opt.is_sparse:
-    raise NotImplementedError("sparse mass matrix (nv >= 60 or jacobian=sparse) is not implemented")
   return m
 
 
@@ -379,12 +378,26 @@ def put_data(mjm, mjd, nworld: int = 1, nconmax: int = -1, njmax: int = -1, devi
       setattr(d, name, t.reshape(getattr(d, name).shape).contiguous())
   qM = getattr(mjd, "qM", None)
   if qM is not None and np.asarray(qM).size == mjm.nM and mjm.nv:
+    # the host qM is MuJoCo's sparse layout; dense models keep it dense,
+    # sparse models keep both the sparse layout and the dense working copy
     M = _dense_qM(mjm, np.asarray(qM))
-    d.qM = _tensor(np.tile(M, (nworld, 1, 1)), "R", device)
+    dense = _tensor(np.tile(M, (nworld, 1, 1)), "R", device)
     try:
-      d.qLD = _tensor(np.tile(np.linalg.cholesky(M), (nworld, 1, 1)), "R", device)
+      dense_L = _tensor(np.tile(np.linalg.cholesky(M), (nworld, 1, 1)), "R", device)
     except np.linalg.LinAlgError:
-      pass
+      dense_L = None
+    if is_sparse(mjm):
+      d.qM = _tensor(np.tile(np.asarray(qM, np.float64).reshape(1, mjm.nM), (nworld, 1, 1)), "R", device)
+      d.qMd = dense
+      if dense_L is not None:
+        d.qLDd = dense_L
+      qLD = getattr(mjd, "qLD", None)
+      if qLD is not None and np.asarray(qLD).size == mjm.nM:
+        d.qLD = _tensor(np.tile(np.asarray(qLD, np.float64).reshape(1, mjm.nM), (nworld, 1, 1)), "R", device)
+    else:
+      d.qM = dense
+      if dense_L is not None:
+        d.qLD = dense_L
   mom = getattr(mjd, "actuator_moment", None)
   if mom is not None and np.asarray(mom).size == mjm.nu * mjm.nv:
     d.actuator_moment = _tensor(np.tile(np.asarray(mom).reshape(mjm.nu, mjm.nv), (nworld, 1, 1)), "R", device)
@@ -447,17 +460,22 @@ def get_data_into(result, mjm, d: types.Data):
   for name in ("xmat", "ximat", "geom_xmat", "site_xmat"):
     setattr(result, name, getattr(d, name).cpu().numpy()[0].reshape(-1, 9).astype(np.float64))
   result.actuator_moment = d.actuator_moment.cpu().numpy()[0].astype(np.float64)
-  qM = d.qM.cpu().numpy()[0]
-  qLD = d.qLD.cpu().numpy()[0]
-  sq, sl = [], []
-  for i in range(mjm.nv):
-    j = i
-    while j >= 0:
-      sq.append(qM[i, j])
-      sl.append(qLD[i, j])
-      j = mjm.dof_parentid[j]
-  result.qM = np.array(sq)
-  result.qLD = np.array(sl)
+  if is_sparse(mjm):
+    # io.py:803-810: the sparse layout is MuJoCo's own
+    result.qM = d.qM.cpu().numpy()[0, 0].astype(np.float64)
+    result.qLD = d.qLD.cpu().numpy()[0, 0].astype(np.float64)
+  else:
+    qM = d.qM.cpu().numpy()[0]
+    qLD = d.qLD.cpu().numpy()[0]
+    sq, sl = [], []
+    for i in range(mjm.nv):
+      j = i
+      while j >= 0:
+        sq.append(qM[i, j])
+        sl.append(qLD[i, j])
+        j = mjm.dof_parentid[j]
+    result.qM = np.array(sq)
+    result.qLD = np.array(sl)
   result.ncon = ncon
   result.nefc = nefc
 

@@ -337,7 +337,39 @@ void com_pos(const Model<T>& m, const Data<T>& d, int w) {
   }
 }
 
-// smooth.py:248-314 crb, dense qM
+// MuJoCo's sparse layout of a dense nv x nv matrix (smooth.py:281-294
+// qM_sparse): row i holds M(i, j) for j = i, parent(i), parent(parent(i)),
+// ... from dof_Madr[i]
+template <class T>
+void dense_to_sparse(const Model<T>& m, const T* M, T* S) {
+  for (int i = 0; i < m.nv; i++) {
+    int a = m.dof_Madr[i];
+    for (int j = i; j >= 0; j = m.dof_parentid[j]) S[a++] = M[i * m.nv + j];
+  }
+}
+
+// smooth.py:317-350 _factor_i_sparse: L'DL of the sparse matrix S into L
+// (unit L below the diagonal, D on it; leaves first, as MuJoCo), and
+// Dinv = 1 / D
+template <class T>
+void factor_sparse(const Model<T>& m, const T* S, T* L, T* Dinv) {
+  std::memcpy(L, S, sizeof(T) * m.nM);
+  for (int k = m.nv - 1; k >= 0; k--) {
+    const int ak = m.dof_Madr[k];
+    int p = 1;
+    for (int i = m.dof_parentid[k]; i >= 0; i = m.dof_parentid[i], p++) {
+      const T tmp = L[ak + p] / L[ak];
+      const int ai = m.dof_Madr[i];
+      const int len = m.dof_Madr[i + 1] - ai;  // i < k <= nv - 1
+      for (int q = 0; q < len; q++) L[ai + q] -= L[ak + p + q] * tmp;
+      L[ak + p] = tmp;
+    }
+  }
+  for (int k = 0; k < m.nv; k++) Dinv[k] = T(1) / L[m.dof_Madr[k]];
+}
+
+// smooth.py:248-314 crb: dense qMd, and for sparse models qM in MuJoCo's
+// sparse layout
 template <class T>
 void crb(const Model<T>& m, const Data<T>& d, int w) {
   const int nb = m.nbody, nv = m.nv;
@@ -361,6 +393,7 @@ void crb(const Model<T>& m, const Data<T>& d, int w) {
       M[j * nv + i] += v;
     }
   }
+  if (m.opt.is_sparse) dense_to_sparse(m, M, d.qM + (size_t)w * m.nM);
 }
 
 // dense Cholesky A = L L' of the n x n block at (off, off) of an ld-strided
@@ -416,6 +449,8 @@ void factor_m(const Model<T>& m, const Data<T>& d, int w) {
   const T* M = d.qMd + (size_t)w * nv * nv;
   T* L = d.qLDd + (size_t)w * nv * nv;
   each_block(m, [&](int off, int n) { cholesky_block(M, L, off, n, nv); });
+  if (m.opt.is_sparse)
+    factor_sparse(m, d.qM + (size_t)w * m.nM, d.qLD + (size_t)w * m.nM, d.qLDiagInv + (size_t)w * nv);
 }
 
 // smooth.py:654-681
@@ -1471,6 +1506,14 @@ void euler(const Model<T>& m, const Data<T>& d, int w) {
     cholesky_block(A.data(), L.data(), off, n, nv);
     cho_solve_block(L.data(), off, n, nv, d.qacc_integration + o, f.data());
   });
+  if (m.opt.is_sparse) {
+    // forward.py:171-193 eulerdamp_sparse: the integration matrix and its
+    // L'DL factor in the sparse layout
+    T* Mi = d.qM_integration + (size_t)w * m.nM;
+    dense_to_sparse(m, A.data(), Mi);
+    factor_sparse(m, Mi, d.qLD_integration + (size_t)w * m.nM, d.qLDiagInv_integration + o);
+    for (int i = 0; i < nv; i++) d.qfrc_integration[o + i] = f[i];
+  }
   advance(m, d, w, d.qacc_integration);
 }
 
