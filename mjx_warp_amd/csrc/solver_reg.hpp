@@ -376,19 +376,9 @@ MJH_DEV void solve_reg_nv(const mjhModel& m, const mjhData& d, float* lds, int r
   }
   const int nst = nr < rj ? nr : rj;
   wave_fence();
-  {
-    // rows [0, nst) as one coalesced block; padding columns [nv, NVC) zero
-    const float* Jw = d.efc_J + (size_t)radr * nv;
-    for (int i = lane; i < nst * nv; i += 32) {
-      const int r = i / nv;
-      s.Jl[r * s.JS + i - r * nv] = Jw[i];
-    }
-    if (nv < NVC)
-      for (int i = lane; i < nst * (NVC - nv); i += 32) {
-        const int r = i / (NVC - nv);
-        s.Jl[r * s.JS + nv + i - r * (NVC - nv)] = 0.0f;
-      }
-  }
+  // rows [0, nst), lanes over columns; padding columns [nv, NVC) zero
+  if (lane < NVC)
+    for (int r = 0; r < nst; r++) s.Jl[r * s.JS + lane] = dof ? s.Jg(r)[lane] : 0.0f;
   if (m.opt.solver == mjhSOL_CG) {
     // first dof of the lane's kinematic-tree block of qLD (io.py:150-162)
     s.blk = lane;
