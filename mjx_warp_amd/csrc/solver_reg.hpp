@@ -58,7 +58,8 @@ struct RegSolver {
   float Jaref[RS], jv[RS], D[RS], force[RS], q0[RS], q1[RS], q2[RS];
 
   int blk;   // CG: first dof of the lane's tree block
-  int nset;  // live row sets, ceil(nr / 32) but at least 1 (team-uniform)
+  int nset;  // live row sets, ceil(nr / 32) but at least 1 (team-uniform); set 0
+             // is always live, which the loops state at compile time
   // the factor in ch is current: Newton's H = M + J' D(active) J depends on
   // the iterate only through the active set, so while no row changes
   // activity the previous factor is the one a rebuild would produce
@@ -113,7 +114,7 @@ struct RegSolver {
     float c = 0.0f;
 #pragma unroll
     for (int s = 0; s < RS; s++) {
-      if (s >= nset) break;
+      if (s > 0 && s >= nset) break;
       const float jar = Jaref[s];
       const bool a = jar < 0.0f;
       force[s] = a ? -1.0f * D[s] * jar : 0.0f;
@@ -123,7 +124,7 @@ struct RegSolver {
     wave_fence();
 #pragma unroll
     for (int s = 0; s < RS; s++) {
-      if (s >= nset) break;
+      if (s > 0 && s >= nset) break;
       fbuf[lane + 32 * s] = force[s];
     }
     wave_fence();
@@ -147,7 +148,7 @@ struct RegSolver {
       unsigned act = 0;
 #pragma unroll
       for (int s = 0; s < RS; s++) {
-        if (s >= nset) break;
+        if (s > 0 && s >= nset) break;
         if (Jaref[s] < 0.0f) act |= 1u << s;
       }
       const float changed = sum32(act != act_prev ? 1.0f : 0.0f);
@@ -167,7 +168,7 @@ struct RegSolver {
       wave_fence();
 #pragma unroll
       for (int s = 0; s < RS; s++) {
-        if (s >= nset) break;
+        if (s > 0 && s >= nset) break;
         fbuf[lane + 32 * s] = Jaref[s] < 0.0f ? D[s] : 0.0f;
       }
       wave_fence();
@@ -203,7 +204,7 @@ struct RegSolver {
     float a = 0.f, b = 0.f, c = 0.f;
 #pragma unroll
     for (int s = 0; s < RS; s++)
-      if (s < nset)
+      if (s == 0 || s < nset)
       if (Jaref[s] + alpha * jv[s] < 0.0f) {
         const V3f p = eval_pt(q0[s], q1[s], q2[s], alpha);
         a += p.a;
@@ -219,7 +220,7 @@ struct RegSolver {
     float pa = 0.f, pb = 0.f, pc = 0.f;
 #pragma unroll
     for (int s = 0; s < RS; s++)
-      if (s < nset)
+      if (s == 0 || s < nset)
       if (Jaref[s] < 0.0f) {
         pa += q0[s];
         pb += q1[s];
@@ -248,7 +249,7 @@ struct RegSolver {
       float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f, s8 = 0.f;
 #pragma unroll
       for (int s = 0; s < RS; s++) {
-        if (s >= nset) break;
+        if (s > 0 && s >= nset) break;
         const float jar = Jaref[s], v = jv[s];
         if (jar + la * v < 0.0f) {
           const V3f p = eval_pt(q0[s], q1[s], q2[s], la);
@@ -291,7 +292,7 @@ struct RegSolver {
       float s0 = 0.f, s1 = 0.f, s2 = 0.f;
 #pragma unroll
       for (int s = 0; s < RS; s++)
-        if (s < nset)
+        if (s == 0 || s < nset)
         if (Jaref[s] + a * jv[s] < 0.0f) {
           s0 += q0[s];
           s1 += q1[s];
@@ -320,7 +321,7 @@ struct RegSolver {
     mv = mrow_dot();
 #pragma unroll
     for (int s = 0; s < RS; s++) {
-      if (s >= nset) break;
+      if (s > 0 && s >= nset) break;
       const float v = jrow_dot(row(s));
       const float jar = Jaref[s], dd = D[s];
       jv[s] = v;
@@ -337,7 +338,7 @@ struct RegSolver {
     Ma += alpha * mv;
 #pragma unroll
     for (int s = 0; s < RS; s++)
-      if (s < nset) Jaref[s] += alpha * jv[s];
+      if (s == 0 || s < nset) Jaref[s] += alpha * jv[s];
     return alpha;
   }
 };
