@@ -5,7 +5,9 @@
  * model description, a data description and a HIP stream.  Both
  * descriptions are plain structs of device pointers and sizes; the caller
  * (normally the Python package over ctypes) owns every buffer, the library
- * allocates nothing inside a call and never synchronises the stream.
+ * allocates no memory inside a call and never synchronises the stream.  It
+ * creates, once per host thread and device, a second stream and two events
+ * that it keeps until the process ends (see mjh_solve).
  *
  * Field names, shapes and meanings follow the reference's Model / Data /
  * Contact / Constraint structs (mujoco_warp/_src/types.py:218-800).  Every
@@ -346,10 +348,11 @@ typedef struct mjhOption {
  * of the global pool, reserved with one atomic per wavefront; world_pair,
  * world_con and world_efc hold (first index, count written) of that block so
  * that later stages find a world's rows without scanning the pool.
- * world_flags marks worlds whose rows exceed the solve launch's row
- * capacity (64 rows for nv <= 32); tail launches of the same step solve and
- * integrate them (the register solver up to 256 rows, then the
- * global-memory solver).  overflow
+ * For nv <= 32 the solve launch holds 64 rows per world; worlds with 65-256
+ * rows are solved (and, in the fused step, integrated) by a second launch
+ * that runs beside it on the library's second stream.  world_flags marks
+ * worlds over every register capacity; the global-memory solver finishes
+ * them in a tail launch of the same step.  overflow
  * counts pairs / contacts / rows dropped because a pool was full (the
  * reference drops contacts silently and writes rows out of bounds,
  * collision_primitive.py:64-65, constraint.py:119,162).
@@ -446,7 +449,7 @@ int mjh_step(const mjhModel* m, const mjhData* d, mjhStream stream);
 /* The same step as mjh_step with the stages fused: for nv <= 32 two
  * launches (every forward stage; solver and integrator) on 32-lane teams,
  * otherwise one launch per step; worlds whose constraint rows exceed the
- * fused solver's capacity finish in tail launches.  No counterpart in the
+ * fused solver's capacity finish in a launch beside it or a tail launch.  No counterpart in the
  * reference; results equal mjh_step's up to floating-point summation
  * order. */
 int mjh_step_fused(const mjhModel* m, const mjhData* d, mjhStream stream);

@@ -229,8 +229,9 @@ class StepGraph:
 
   The counterpart of the reference benchmark's captured step
   (test_util.py:99-118, wp.capture_begin / capture_launch): the step's
-  launches are recorded on a capture stream (the library allocates nothing
-  and records no memset nodes, so a step is capturable) and `replay()`
+  launches are recorded on a capture stream (the library allocates no
+  memory and records no memset nodes, so a step is capturable; its
+  big-world solve forks onto a second stream and joins back) and `replay()`
   launches them on the current stream.  Replays read and write the same
   Model / Data tensors that were captured; `d.time` is advanced on the
   host by each replay, as `step` does.

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 
 #include "collision.hpp"
@@ -80,23 +81,41 @@ MJH_DEV bool solve_team(const mjhModel& m, const mjhData& d, float* l, const Tea
   if constexpr (T == kRegTeam) return solve_reg<2>(m, d, l, tune.rj, t);
   else return solve_lds(m, d, l, tune.rmax, tune.rj, t);
 }
-STAGE(StSolveTeam, if (!solve_team(m, d, l, t, tune) && t.lane == 0) d.world_flags[t.world] |= kFlagDeferred)
-// register tail (32-lane teams): flagged worlds of up to kRegRows rows get
-// the register solver with kRegSets row sets (and, when `integrate` is set,
-// the integrator), then the flag is cleared; larger worlds stay flagged
-template <bool integrate>
-struct StTailReg {
+// Worlds the 32-lane solve launches leave to the big-world launch: more
+// rows than two row sets hold, at most kRegRows.
+MJH_DEV bool big_world(const mjhData& d, int w) {
+  if (d.njmax == 0) return false;
+  const int nr = d.world_efc[2 * w + 1];
+  return nr > 2 * kRegTeam && nr <= kRegRows;
+}
+template <int T>
+MJH_DEV bool left_to_big(const mjhData& d, const Team<T>& t) {
+  if constexpr (T == kRegTeam) return t.valid && big_world(d, t.world);
+  else return false;
+}
+STAGE(StSolveTeam, if (left_to_big(d, t)) return;
+      if (!solve_team(m, d, l, t, tune) && t.lane == 0) d.world_flags[t.world] |= kFlagDeferred)
+// big worlds (32-lane teams, big_world): the register solver with kRegSets
+// row sets; with `step` set also what the fused solve launch does around it
+// (factorisation, smooth acceleration, integrator).  It touches no other
+// world, so it can run beside the solve launch, which skips these worlds.
+template <bool step>
+struct StBigWorlds {
   template <int T>
   MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
-    static_assert(T == kRegTeam, "register tail runs on 32-lane teams");
-    if (!t.valid || !(d.world_flags[t.world] & kFlagDeferred)) return;
-    if (!solve_reg<kRegSets>(m, d, l, tune.rj, t)) return;
+    static_assert(T == kRegTeam, "big worlds run on 32-lane teams");
+    if (!t.valid || !big_world(d, t.world)) return;
+    if (step) {
+      factor_m(m, d, l, t);
+      sync();
+      fwd_acceleration_solve(m, d, l, t);
+    }
+    solve_reg<kRegSets>(m, d, l, tune.rj, t);
     sync();
-    if (integrate) {
+    if (step) {
       if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
       else implicit(m, d, l, t);
     }
-    if (t.lane == 0) d.world_flags[t.world] &= ~kFlagDeferred;
   }
 };
 
@@ -172,6 +191,7 @@ struct StFusedForward {
 struct StFusedSolveIntegrate {
   template <int T>
   MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
+    if (left_to_big(d, t)) return;  // StBigWorlds<true> steps it
     factor_m(m, d, l, t);
     sync();
     fwd_acceleration_solve(m, d, l, t);
@@ -306,10 +326,76 @@ int lds_tail_reg(const mjhModel& m) {
   return ((a > b ? a : b) + 3) & ~3;
 }
 
+// A second stream per device and host thread for work that may run beside
+// the caller's stream (default priority; ordered against it with events).
+// Null when it cannot be created: that work then runs in order on the
+// caller's stream.  Kept until the process ends.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+SideStream* side_stream() {
+  thread_local std::map<int, SideStream> per_device;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  SideStream& ss = per_device[dev];
+  if (!ss.s) {
+    // Creation may happen while the caller captures a graph: relaxed mode
+    // keeps it from invalidating that capture.
+    hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
+    const bool ok = hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) == hipSuccess &&
+                    hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) == hipSuccess &&
+                    hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) == hipSuccess;
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
+    if (!ok) {
+      (void)hipGetLastError();
+      if (ss.join) (void)hipEventDestroy(ss.join);
+      if (ss.fork) (void)hipEventDestroy(ss.fork);
+      if (ss.s) (void)hipStreamDestroy(ss.s);
+      ss = SideStream{};
+      return nullptr;
+    }
+  }
+  return &ss;
+}
+
+#define TRY(x)                     \
+  do {                             \
+    int _rc = (x);                 \
+    if (_rc != MJH_OK) return _rc; \
+  } while (0)
+
 int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(MJH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
   return MJH_OK;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) return fail(MJH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return MJH_OK;
+}
+
+// Runs side(stream) beside main(s): side on the side stream, forked after
+// the work already queued on s and joined before what s runs next; in order
+// on s when there is no side stream.
+template <class Side, class Main>
+int beside(hipStream_t s, Side side, Main main) {
+  SideStream* ss = side_stream();
+  if (!ss) {
+    TRY(main(s));
+    return side(s);
+  }
+  TRY(check_hip(hipEventRecord(ss->fork, s), "fork"));
+  TRY(check_hip(hipStreamWaitEvent(ss->s, ss->fork, 0), "fork"));
+  TRY(side(ss->s));
+  TRY(main(s));
+  TRY(check_hip(hipEventRecord(ss->join, ss->s), "join"));
+  return check_hip(hipStreamWaitEvent(s, ss->join, 0), "join");
 }
 
 template <class S, int TEAM>
@@ -339,11 +425,7 @@ int reset(const mjhData& d, hipStream_t s, int which) {
   return check_launch("reset");
 }
 
-#define TRY(x)                     \
-  do {                             \
-    int _rc = (x);                 \
-    if (_rc != MJH_OK) return _rc; \
-  } while (0)
+
 
 int do_collision(const mjhModel& m, const mjhData& d, hipStream_t s) {
   TRY(launch<StBroadphase>(m, d, s, lds_broadphase(m), "broadphase"));
@@ -352,9 +434,13 @@ int do_collision(const mjhModel& m, const mjhData& d, hipStream_t s) {
 
 int do_solve(const mjhModel& m, const mjhData& d, hipStream_t s) {
   if (reg_path(m)) {
-    TRY((launch_team<StSolveTeam, kRegTeam>(m, d, s, lds_solve_reg(m), "solve")));
     const Tune tt = tune_tail_reg(m);
-    TRY((launch_team<StTailReg<false>, kRegTeam>(m, d, s, lds_tail_reg(m), "solve_tail_reg", &tt)));
+    TRY(beside(
+        s,
+        [&](hipStream_t q) {
+          return launch_team<StBigWorlds<false>, kRegTeam>(m, d, q, lds_tail_reg(m), "solve_big_worlds", &tt);
+        },
+        [&](hipStream_t q) { return launch_team<StSolveTeam, kRegTeam>(m, d, q, lds_solve_reg(m), "solve"); }));
   } else {
     TRY((launch_team<StSolveTeam, kTeam>(m, d, s, lds_solve(m), "solve")));
   }
@@ -405,9 +491,15 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
   TRY(reset(d, s, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3));
   if (reg_path(m)) {
     TRY((launch_team<StFusedForward, kRegTeam>(m, d, s, lds_forward(m), "step_fused_forward")));
-    TRY((launch_team<StFusedSolveIntegrate, kRegTeam>(m, d, s, lds_solve_reg(m), "step_fused_solve")));
     const Tune tt = tune_tail_reg(m);
-    TRY((launch_team<StTailReg<true>, kRegTeam>(m, d, s, lds_tail_reg(m), "step_tail_reg", &tt)));
+    TRY(beside(
+        s,
+        [&](hipStream_t q) {
+          return launch_team<StBigWorlds<true>, kRegTeam>(m, d, q, lds_tail_reg(m), "step_big_worlds", &tt);
+        },
+        [&](hipStream_t q) {
+          return launch_team<StFusedSolveIntegrate, kRegTeam>(m, d, q, lds_solve_reg(m), "step_fused_solve");
+        }));
   } else {
     TRY((launch_team<StFusedStep, kTeam>(m, d, s, lds_step(m), "step_fused")));
   }

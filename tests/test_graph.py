@@ -5,7 +5,9 @@ Two Data objects start from the same seeded state; one advances with eager
 steps.  Every replay is checked against the eager result, with eager work
 and synchronisations between the replays.  Each world's computation does
 not depend on where its block lands in the contact / row pools, so the two
-must agree bit for bit.
+must agree bit for bit.  The lying-humanoid case has worlds with more than
+64 constraint rows, which a second launch on a side stream solves beside the
+main solve launch: the graph then holds a fork and a join.
 """
 
 import numpy as np
@@ -20,11 +22,11 @@ import helpers
 pytestmark = pytest.mark.gpu
 
 
-def _state(mjm, nworld, seed):
+def _state(mjm, nworld, seed, noise=0.05):
   rng = np.random.default_rng(seed)
   d = mjw.make_data(mjm, nworld=nworld, nconmax=64 * nworld, njmax=277 * nworld)
   q = np.tile(mjm.key_qpos[0], (nworld, 1))
-  q[:, 7:] += rng.normal(0, 0.05, (nworld, mjm.nq - 7))
+  q[:, 7:] += rng.normal(0, noise, (nworld, mjm.nq - 7))
   helpers.set_field(d, "qpos", q)
   helpers.set_field(d, "qvel", rng.normal(0, 0.5, (nworld, mjm.nv)))
   helpers.set_field(d, "ctrl", rng.uniform(-1, 1, (nworld, mjm.nu)))
@@ -50,3 +52,30 @@ def test_graph_replays_match_eager_steps(fused):
       b = helpers.data_numpy(d_eager, f)
       assert np.array_equal(a, b), (rep, f, np.abs(a - b).max())
     assert abs(d_graph.time - d_eager.time) < 1e-12
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_graph_with_big_worlds_matches_eager(fused):
+  mjm = models.humanoid(iterations=10, ls_iterations=20)
+  q0 = mjm.key_qpos[0].copy()
+  q0[:3] = [0.0, 0.0, 0.05]
+  q0[3:7] = [np.cos(np.pi / 4), 0.0, np.sin(np.pi / 4), 0.0]
+  q0[7:] = 0.0
+  mjm.key_qpos[0] = q0
+  m = mjw.put_model(mjm)
+  d_eager = _state(mjm, 64, 5, noise=0.2)
+  d_graph = _state(mjm, 64, 5, noise=0.2)
+  step = mjw.step_fused if fused else mjw.step
+  g = mjw.StepGraph(m, d_graph, nstep=1, fused=fused)
+  for rep in range(3):
+    step(m, d_eager)
+    torch.cuda.synchronize()
+    if rep == 0:
+      rows = np.array([n for _, n in helpers.world_blocks(helpers.data_numpy(d_eager, "world_efc"), 64)])
+      assert rows.max() > 64 and rows.min() <= 64, rows
+    g.replay()
+    torch.cuda.synchronize()
+    for f in ("qpos", "qvel", "qacc", "qacc_warmstart"):
+      a = helpers.data_numpy(d_graph, f)
+      b = helpers.data_numpy(d_eager, f)
+      assert np.array_equal(a, b), (rep, f, np.abs(a - b).max())
