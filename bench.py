@@ -13,7 +13,10 @@ kinematics, inertia, collision, constraint rows, solver, integration) plus
 the ctrl resample.  `value` is B * steps / wall time of the timed loop with
 all state resident in HBM; the fused step (a forward launch and a
 solve+integrate launch, plus a tail for worlds over the solver's row
-capacity) is the headline, the plain per-stage step is reported beside it.
+capacity) is the headline, the plain per-stage step is reported beside it,
+and so are the batched-persistent form (10 fused steps captured as one HIP
+graph, replayed) and forward + adjoint world-steps/s of a differentiable
+50-step rollout of 1024 humanoids (`forward_adjoint`).
 `roofline.traffic` is the fused step's HBM traffic measured with rocprofv3
 for the default configuration (profiles/step_humanoid_b8192_newton10.md).
 
@@ -135,6 +138,94 @@ def time_form(args, fused: bool):
   return mjm, res
 
 
+def time_rollout_graph(args, nstep: int = 10):
+  """The batched-persistent form: nstep fused steps captured once as a HIP
+  graph and replayed (ctrl resampled between replays)."""
+  import torch
+
+  import mjx_warp_amd as mjw
+
+  _, m, d = setup(args)
+  gen = torch.Generator(device="cuda")
+  gen.manual_seed(1)
+  graph = mjw.StepGraph(m, d, nstep=nstep, fused=True)
+  u = torch.empty_like(d.ctrl)
+  reps = max(1, args.steps // nstep)
+
+  def one():
+    torch.rand(u.shape, generator=gen, device="cuda", out=u)
+    d.ctrl.copy_(u.mul_(2.0).sub_(1.0))
+    graph.replay()
+
+  for _ in range(max(1, args.warmup // nstep)):
+    one()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(reps):
+    one()
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  return {"value": args.batch * reps * nstep / wall, "ms_per_step": 1000.0 * wall / (reps * nstep),
+          "steps_per_graph": nstep, "replays": reps}
+
+
+def time_adjoint(args, nworld: int = 1024, nstep: int = 50, eps: float = 1e-3, reps: int = 2):
+  """Forward + adjoint world-steps/s of the differentiable rollout: the
+  humanoid (gear 100, implicitfast, Newton) from perturbed squat states,
+  nworld worlds, nstep steps, ctrl ~ U(-1, 1) [nstep, nworld, 21], loss
+  sum_t (torso z - 1.3)^2 + 1e-3 |ctrl|^2, gradients with respect to ctrl and
+  qvel0 through the finite-difference transition Jacobians (centered, eps)
+  and the adjoint recursion."""
+  import torch
+
+  import mjx_warp_amd as mjw
+  from mjx_warp_amd import derivative, models
+
+  mjm = models.humanoid(gear=100.0, iterations=args.iterations, ls_iterations=args.ls_iterations)
+  m = mjw.put_model(mjm)
+  r = derivative.Rollout(mjm, m, nworld, nstep, nconmax=64 * nworld, njmax=277 * nworld, eps=eps)
+  q0, v0 = models.humanoid_rollout_state(mjm, nworld)
+  gen = torch.Generator(device="cuda")
+  gen.manual_seed(1)
+  ctrl = torch.rand((nstep, nworld, mjm.nu), generator=gen, device="cuda") * 2.0 - 1.0
+  tq0 = torch.tensor(q0, dtype=torch.float32, device="cuda")
+  tv0 = torch.tensor(v0, dtype=torch.float32, device="cuda")
+
+  def fwd_adj():
+    qv = tv0.clone().requires_grad_(True)
+    u = ctrl.clone().requires_grad_(True)
+    qpos, _ = r(tq0, qv, u)
+    loss = ((qpos[..., 2] - 1.3) ** 2).sum() + 1e-3 * (u**2).sum()
+    loss.backward()
+    return u.grad, qv.grad
+
+  def fwd():
+    with torch.no_grad():
+      r.forward(tq0, tv0, ctrl)
+
+  fwd_adj()
+  fwd()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(reps):
+    fwd()
+  torch.cuda.synchronize()
+  t_fwd = (time.perf_counter() - t0) / reps
+  t0 = time.perf_counter()
+  for _ in range(reps):
+    gu, gv = fwd_adj()
+  torch.cuda.synchronize()
+  t_all = (time.perf_counter() - t0) / reps
+  finite = bool(torch.isfinite(gu).all().item() and torch.isfinite(gv).all().item())
+  return {"value": nworld * nstep / t_all, "unit": "world-steps/s", "forward_value": nworld * nstep / t_fwd,
+          "cost_ratio": t_all / t_fwd, "ms_forward": 1000.0 * t_fwd, "ms_forward_adjoint": 1000.0 * t_all,
+          "nworld": nworld, "nstep": nstep, "eps": eps, "fd_worlds": nworld * r.tfd.per_world,
+          "gradients_finite": finite,
+          "workload": "humanoid 27-dof rollout, gradients of sum_t (z_t - 1.3)^2 + 1e-3 |u_t|^2 w.r.t. "
+                      "ctrl [50, 1024, 21] and qvel0 [1024, 27]; centered finite-difference transition "
+                      "Jacobians (eps 1e-3) + adjoint recursion"}
+
+
 def cpu_baseline(args, target_s: float = 15.0):
   """The oracle (FP32 CPU restatement, OpenMP over worlds) on this host."""
   import numpy as np
@@ -201,6 +292,8 @@ def main():
   ap.add_argument("--ls_iterations", type=int, default=20)
   ap.add_argument("--no-plain", action="store_true", help="skip timing the plain per-stage step")
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+  ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint rollout")
+  ap.add_argument("--no-rollout-graph", action="store_true", help="skip the graph-replayed rollout")
   ap.add_argument("--graph", action="store_true", help="replay a HIP graph of one captured step")
   ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                   help="only run a plain copy of MIB MiB (counter calibration) and exit")
@@ -224,6 +317,14 @@ def main():
   if not args.no_plain:
     _, plain = time_form(args, fused=False)
     log(f"plain: {plain['value']:.4g} world-steps/s, {plain['ms_per_step']:.3f} ms/step")
+  rgraph = None
+  if not args.no_rollout_graph:
+    rgraph = time_rollout_graph(args)
+    log(f"rollout graph: {rgraph['value']:.4g} world-steps/s")
+  adj = None
+  if not args.no_adjoint:
+    adj = time_adjoint(args)
+    log(f"forward+adjoint: {adj['value']:.4g} world-steps/s (cost ratio {adj['cost_ratio']:.1f})")
   bytes_ws = algorithmic_bytes(mjm, fused["ncon_w"], fused["nefc_w"])
   achieved = bytes_ws * args.batch / (fused["kernel_ms"] * 1e-3) / 1e9
   base = None
@@ -260,6 +361,8 @@ def main():
     "state_finite": fused["finite"],
     "plain_step": None if plain is None else {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
                                               "kernel_ms": plain["kernel_ms"]},
+    "rollout_graph": rgraph,
+    "forward_adjoint": adj,
   }
   print(json.dumps(out), flush=True)
 

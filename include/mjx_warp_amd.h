@@ -456,6 +456,39 @@ int mjh_step_fused(const mjhModel* m, const mjhData* d, mjhStream stream);
 /* forward.py:43-162 and 682-695 for nstep steps in one call */
 int mjh_rollout(const mjhModel* m, const mjhData* d, int nstep, int fused, mjhStream stream);
 
+/* ---- transition derivatives and the adjoint ----
+ * The reference has no gradients (its kernels overwrite arrays in place;
+ * collision_convex.py:22 disables backward code generation); these follow
+ * MuJoCo's mjd_transitionFD (engine_derivative_fd.c) instead.  State
+ * x = (dq [nv], qvel [nv], act [na]), nx = 2 nv + na, with dq the tangent
+ * space of qpos (local-frame rotation vectors for free / ball joints, as
+ * mj_integratePos / mj_differentiatePos); input u = ctrl [nu]. */
+
+/* Finite-difference Jacobians of one fused step at d's state, per world:
+ * A [nworld, nx, nx] = dx'/dx, B [nworld, nx, nu] = dx'/du.  fd is a Data
+ * of the same model with fd->nworld = d->nworld * (1 + S (nx + nu)) worlds
+ * (S = 2 if centered, else 1), used as scratch: world w*K holds the
+ * nominal state, w*K + 1 + c*S + s column c's perturbation by +eps (s = 0)
+ * or -eps (s = 1).  d is read and not stepped.  A control column uses only
+ * the sides that stay inside ctrlrange (zero column if neither does), as
+ * mjd_transitionFD does. */
+int mjh_transition_fd(const mjhModel* m, const mjhData* d, const mjhData* fd, float eps, int centered, float* A,
+                      float* B, mjhStream stream);
+/* One step of the adjoint recursion at d's state (d->qpos is used to turn
+ * a raw qpos gradient into the tangent space):
+ *   lam = A' lam_next + g_x,   grad_ctrl = B' lam_next,
+ * lam / lam_next [nworld, nx], g_x from g_qpos [nworld, nq] (raw
+ * coordinates), g_qvel [nworld, nv], g_act [nworld, na].  Null gradients and
+ * a null lam_next count as zero; grad_ctrl may be null. */
+int mjh_adjoint_step(const mjhModel* m, const mjhData* d, const float* A, const float* B, const float* g_qpos,
+                     const float* g_qvel, const float* g_act, const float* lam_next, float* lam, float* grad_ctrl,
+                     mjhStream stream);
+/* lam [nworld, nx] as gradients with respect to d's raw state: g_qpos
+ * [nworld, nq] (a rotation's tangent gradient g becomes 2 q (0, g),
+ * orthogonal to q), g_qvel [nworld, nv], g_act [nworld, na]. */
+int mjh_adjoint_state_grad(const mjhModel* m, const mjhData* d, const float* lam, float* g_qpos, float* g_qvel,
+                           float* g_act, mjhStream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,12 +3,15 @@
 // Every stage kernel is launched with 64-lane workgroups (one wavefront),
 // each holding 64/TEAM worlds; the grid covers nworld.  The plain step
 // (mjh_step) launches one kernel per reference stage; the fused step
-// (mjh_step_fused) runs every stage of a world inside one launch.  Both use
-// the LDS-resident solver; worlds whose constraint rows exceed its row
-// capacity are flagged and finished by a tail launch of the global-memory
-// solver (and, for the fused step, the integrator).
+// (mjh_step_fused) runs every stage of a world inside one launch (two on
+// 32-lane teams).  Both use the register (nv <= 32) or LDS solver; on
+// 32-lane teams worlds with 65-256 rows are solved by a second launch beside
+// the main solve launch, and worlds over every capacity are flagged and
+// finished by a tail launch of the global-memory solver (and, for the fused
+// step, the integrator).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -17,6 +20,7 @@
 
 #include "collision.hpp"
 #include "constraint.hpp"
+#include "derivative.hpp"
 #include "forward.hpp"
 #include "smooth.hpp"
 #include "solver.hpp"
@@ -248,6 +252,28 @@ __global__ void __launch_bounds__(64) k_solve_m(mjhModel m, mjhData d, float* x,
 __global__ void __launch_bounds__(64) k_xfrc(mjhModel m, mjhData d, float* qfrc) {
   Team<kTeam> t(d.nworld);
   xfrc_accumulate(m, d, qfrc, t);
+}
+
+// transition derivatives and the adjoint recursion
+__global__ void __launch_bounds__(64) k_fd_perturb(mjhModel m, mjhData d, mjhData fd, float eps, int centered) {
+  Team<kTeam> t(fd.nworld);
+  fd_perturb(m, d, fd, eps, centered, t);
+}
+__global__ void __launch_bounds__(64) k_fd_difference(mjhModel m, mjhData fd, int nworld, float eps, int centered,
+                                                      float* A, float* B) {
+  Team<kTeam> t(nworld * (fd_nx(m) + m.nu));
+  fd_difference(m, fd, nworld, eps, centered, A, B, t);
+}
+__global__ void __launch_bounds__(64) k_adjoint_step(mjhModel m, mjhData d, const float* A, const float* B,
+                                                     const float* g_qpos, const float* g_qvel, const float* g_act,
+                                                     const float* lam_next, float* lam, float* grad_ctrl) {
+  Team<kTeam> t(d.nworld);
+  adjoint_step(m, d, A, B, g_qpos, g_qvel, g_act, lam_next, lam, grad_ctrl, t);
+}
+__global__ void __launch_bounds__(64) k_adjoint_state_grad(mjhModel m, mjhData d, const float* lam, float* g_qpos,
+                                                           float* g_qvel, float* g_act) {
+  Team<kTeam> t(d.nworld);
+  adjoint_state_grad(m, d, lam, g_qpos, g_qvel, g_act, t);
 }
 
 int lds_linalg(const mjhModel& m) { return m.nv * m.nv + m.nv; }
@@ -617,6 +643,48 @@ int mjh_xfrc_accumulate(const mjhModel* m, const mjhData* d, float* qfrc, mjhStr
   if (d->nworld <= 0) return MJH_OK;
   hipLaunchKernelGGL(k_xfrc, dim3(d->nworld), dim3(64), 0, (hipStream_t)stream, *m, *d, qfrc);
   return check_launch("xfrc_accumulate");
+}
+
+int mjh_transition_fd(const mjhModel* m, const mjhData* d, const mjhData* fd, float eps, int centered, float* A,
+                      float* B, mjhStream stream) {
+  TRY(mjh_check(m, d));
+  TRY(mjh_check(m, fd));
+  if (!(eps > 0.0f) || !std::isfinite(eps)) return fail(MJH_ERR_INVALID, "transition_fd: eps must be positive");
+  const int nx = 2 * m->nv + m->na, nc = nx + m->nu;
+  const long long K = 1 + (centered ? 2LL : 1LL) * nc;
+  if ((long long)fd->nworld != K * d->nworld)
+    return fail(MJH_ERR_INVALID, "transition_fd: fd must hold " + std::to_string(K) + " worlds per world of d");
+  if ((long long)d->nworld * nc > 0x7fffffffLL) return fail(MJH_ERR_CAPACITY, "transition_fd: too many columns");
+  if (!A || (m->nu > 0 && !B)) return fail(MJH_ERR_INVALID, "transition_fd: A and B must be set");
+  if (d->nworld <= 0) return MJH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fd_perturb, dim3(fd->nworld), dim3(64), 0, s, *m, *d, *fd, eps, centered ? 1 : 0);
+  TRY(check_launch("transition_fd: perturb"));
+  TRY(do_step_fused(*m, *fd, s));
+  hipLaunchKernelGGL(k_fd_difference, dim3(d->nworld * nc), dim3(64), 0, s, *m, *fd, d->nworld, eps,
+                     centered ? 1 : 0, A, B);
+  return check_launch("transition_fd: difference");
+}
+int mjh_adjoint_step(const mjhModel* m, const mjhData* d, const float* A, const float* B, const float* g_qpos,
+                     const float* g_qvel, const float* g_act, const float* lam_next, float* lam, float* grad_ctrl,
+                     mjhStream stream) {
+  if (!m || !d) return fail(MJH_ERR_INVALID, "null model or data");
+  if (!lam || (lam_next && (!A || (m->nu > 0 && grad_ctrl && !B))))
+    return fail(MJH_ERR_INVALID, "adjoint_step: lam (and A, B with lam_next) must be set");
+  if (d->nworld <= 0) return MJH_OK;
+  hipLaunchKernelGGL(k_adjoint_step, dim3(d->nworld), dim3(64), 0, (hipStream_t)stream, *m, *d, A, B, g_qpos, g_qvel,
+                     g_act, lam_next, lam, grad_ctrl);
+  return check_launch("adjoint_step");
+}
+int mjh_adjoint_state_grad(const mjhModel* m, const mjhData* d, const float* lam, float* g_qpos, float* g_qvel,
+                           float* g_act, mjhStream stream) {
+  if (!m || !d) return fail(MJH_ERR_INVALID, "null model or data");
+  if (!lam || !g_qpos || !g_qvel || (m->na > 0 && !g_act))
+    return fail(MJH_ERR_INVALID, "adjoint_state_grad: lam and the gradients must be set");
+  if (d->nworld <= 0) return MJH_OK;
+  hipLaunchKernelGGL(k_adjoint_state_grad, dim3(d->nworld), dim3(64), 0, (hipStream_t)stream, *m, *d, lam, g_qpos,
+                     g_qvel, g_act);
+  return check_launch("adjoint_state_grad");
 }
 
 }  // extern "C"

@@ -250,10 +250,19 @@ MJH_DEV void euler_sparse_outputs(const mjhModel& m, const mjhData& d, float* ld
   factor_sparse(m, Mi, WPTR(d.qLD_integration, w, nM), WPTR(d.qLDiagInv_integration, w, nv), lds, t);
 }
 
+// Whether the integrator advances with qacc itself (no implicit term to
+// solve), else with qacc_integration.
+MJH_DEV bool advances_with_qacc(const mjhModel& m) {
+  if (m.opt.integrator == mjhINT_EULER) return m.opt.disableflags & mjhDSBL_EULERDAMP;
+  const bool passive_on = !(m.opt.disableflags & mjhDSBL_PASSIVE);
+  const bool act_on = !(m.opt.disableflags & mjhDSBL_ACTUATION) && m.actuator_affine_bias_gain;
+  return !passive_on && !act_on;
+}
+
 template <int TEAM>
 MJH_DEV void euler(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int w = t.world, nv = m.nv;
-  if (m.opt.disableflags & mjhDSBL_EULERDAMP) {
+  if (advances_with_qacc(m)) {
     advance(m, d, d.qacc, t);
     sync();
     return;
@@ -316,7 +325,7 @@ MJH_DEV void implicit(const mjhModel& m, const mjhData& d, float* lds, const Tea
   const int w = t.world, nv = m.nv, nu = m.nu;
   const bool passive_on = !(m.opt.disableflags & mjhDSBL_PASSIVE);
   const bool act_on = !(m.opt.disableflags & mjhDSBL_ACTUATION) && m.actuator_affine_bias_gain;
-  if (!passive_on && !act_on) {
+  if (advances_with_qacc(m)) {
     advance(m, d, d.qacc, t);
     sync();
     return;

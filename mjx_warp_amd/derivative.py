@@ -1,0 +1,194 @@
+"""Transition derivatives of the step and a differentiable rollout.
+
+The reference has no gradients: its kernels overwrite arrays in place and
+use integer atomics, and collision_convex.py:22 turns Warp's backward code
+generation off.  This module follows MuJoCo's own derivative interface,
+mjd_transitionFD (engine_derivative_fd.c), on the GPU:
+
+* `TransitionFD(mjm, m, nworld, ...)` -- `A, B = tfd(d)` gives, for every
+  world of d, A = dx'/dx [nx, nx] and B = dx'/du [nx, nu] of one step, with
+  x = (dq [nv], qvel [nv], act [na]) (dq: tangent space of qpos, local-frame
+  rotation vectors for free / ball joints) and u = ctrl.  All perturbed
+  copies of all worlds are stepped in one batched step of a second Data of
+  nworld * (1 + 2 (nx + nu)) worlds (centered differences; + (nx + nu) for
+  forward differences), the fused step's own kernels, so a Jacobian costs
+  one step of that batch.
+* `Rollout(mjm, m, nworld, nstep, ...)` -- `qpos, qvel = r(qpos0, qvel0,
+  ctrl)` runs nstep steps and is differentiable with respect to qpos0, qvel0
+  and ctrl [nstep, nworld, nu] through `torch.autograd`.  The forward pass
+  keeps each step's inputs (qpos, qvel, act, qacc_warmstart: a few hundred
+  bytes per world-step, so every step is a checkpoint); the backward pass
+  walks the steps in reverse, computes each step's (A, B) at its saved
+  inputs and runs the adjoint recursion
+      lam_T = g(x_T),  lam_t = A_t' lam_{t+1} + g(x_t),
+      dL/du_t = B_t' lam_{t+1},
+  on the GPU (mjh_adjoint_step), with g the loss gradient the caller's
+  autograd graph hands back for each returned state.
+
+Gradients are those of the discrete FP32 step at the active set of the
+nominal state; a perturbation that switches a contact or limit on or off
+inside +-eps makes the corresponding column a secant, as mjd_transitionFD's
+does.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _native
+from . import api
+from . import io
+from .types import Data, Model
+
+
+def _ptr(t):
+  return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def nx_of(mjm) -> int:
+  return 2 * int(mjm.nv) + int(mjm.na)
+
+
+class TransitionFD:
+  """Finite-difference transition Jacobians of the fused step for every
+  world of a Data of `nworld` worlds (mjd_transitionFD counterpart)."""
+
+  def __init__(self, mjm, m: Model, nworld: int, nconmax: int = -1, njmax: int = -1, eps: float = 1e-3,
+               centered: bool = True, device=None):
+    if m.opt.integrator not in (0, 3):
+      raise NotImplementedError(f"integrator {m.opt.integrator} not implemented.")
+    self.m = m
+    self.nworld = int(nworld)
+    self.eps = float(eps)
+    self.centered = bool(centered)
+    self.nx, self.nu = nx_of(mjm), int(mjm.nu)
+    self.per_world = 1 + (2 if centered else 1) * (self.nx + self.nu)
+    nconmax = 512 if nconmax == -1 else nconmax
+    njmax = 512 if njmax == -1 else njmax
+    # the same per-world capacity as the nominal Data
+    k = self.per_world
+    self.fd = io.make_data(mjm, nworld=k * self.nworld, nconmax=k * nconmax, njmax=k * njmax, device=device)
+    dev = self.fd.qpos.device
+    self.A = torch.zeros((self.nworld, self.nx, self.nx), dtype=torch.float32, device=dev)
+    self.B = torch.zeros((self.nworld, self.nx, self.nu), dtype=torch.float32, device=dev)
+    self._fd_struct = _native.data_struct(m, self.fd)
+
+  def __call__(self, d: Data):
+    """(A, B) at d's current state; d is read, not stepped.  The returned
+    tensors are overwritten by the next call."""
+    if d.nworld != self.nworld:
+      raise ValueError(f"TransitionFD was built for {self.nworld} worlds, got {d.nworld}")
+    _native.call("mjh_transition_fd", self.m, d, ctypes.byref(self._fd_struct), ctypes.c_float(self.eps),
+                 ctypes.c_int(int(self.centered)), _ptr(self.A), _ptr(self.B))
+    return self.A, self.B
+
+
+def adjoint_step(m: Model, d: Data, A, B, lam, g_qpos=None, g_qvel=None, g_act=None, lam_next=None, grad_ctrl=None):
+  """lam = A' lam_next + g(x) at d's state; grad_ctrl = B' lam_next."""
+  _native.call("mjh_adjoint_step", m, d, _ptr(A), _ptr(B), _ptr(g_qpos), _ptr(g_qvel), _ptr(g_act), _ptr(lam_next),
+               _ptr(lam), _ptr(grad_ctrl))
+
+
+def adjoint_state_grad(m: Model, d: Data, lam, g_qpos, g_qvel, g_act=None):
+  """Splits lam [nworld, nx] into gradients with respect to d's raw qpos,
+  qvel and act."""
+  _native.call("mjh_adjoint_state_grad", m, d, _ptr(lam), _ptr(g_qpos), _ptr(g_qvel), _ptr(g_act))
+
+
+class Rollout:
+  """Differentiable `nstep`-step rollout of `nworld` worlds:
+  `qpos, qvel = r(qpos0, qvel0, ctrl)` with qpos [nstep, nworld, nq] and
+  qvel [nstep, nworld, nv] the states after each step.  Inputs other than
+  (qpos, qvel, ctrl) -- qfrc_applied, xfrc_applied, mocap -- are read from
+  `r.d`, set them there; act and qacc_warmstart start from `act0` /
+  `warmstart0` (zeros by default)."""
+
+  def __init__(self, mjm, m: Model, nworld: int, nstep: int, nconmax: int = -1, njmax: int = -1, eps: float = 1e-3,
+               centered: bool = True, device=None):
+    self.m = m
+    self.nworld, self.nstep = int(nworld), int(nstep)
+    self.d = io.make_data(mjm, nworld=nworld, nconmax=nconmax, njmax=njmax, device=device)
+    self.tfd = TransitionFD(mjm, m, nworld, nconmax, njmax, eps, centered, device)
+    self.nq, self.nv, self.na, self.nu = int(mjm.nq), int(mjm.nv), int(mjm.na), int(mjm.nu)
+    dev = self.d.qpos.device
+    z = lambda *s: torch.zeros(s, dtype=torch.float32, device=dev)
+    T, W = self.nstep, self.nworld
+    # inputs of each step (checkpoints)
+    self.s_qpos, self.s_qvel = z(T, W, self.nq), z(T, W, self.nv)
+    self.s_act, self.s_warm = z(T, W, self.na), z(T, W, self.nv)
+    self.lam = z(W, nx_of(mjm))
+    self.lam_next = z(W, nx_of(mjm))
+
+  def _load(self, t, ctrl):
+    d = self.d
+    d.qpos.copy_(self.s_qpos[t])
+    d.qvel.copy_(self.s_qvel[t])
+    d.act.copy_(self.s_act[t])
+    d.qacc_warmstart.copy_(self.s_warm[t])
+    d.ctrl.copy_(ctrl[t])
+
+  def forward(self, qpos0, qvel0, ctrl, act0=None, warmstart0=None):
+    """Runs the rollout without recording anything for autograd."""
+    d, T = self.d, self.nstep
+    if tuple(ctrl.shape) != (T, self.nworld, self.nu):
+      raise ValueError(f"ctrl must be [{T}, {self.nworld}, {self.nu}], got {tuple(ctrl.shape)}")
+    d.qpos.copy_(qpos0)
+    d.qvel.copy_(qvel0)
+    d.act.copy_(act0) if act0 is not None else d.act.zero_()
+    d.qacc_warmstart.copy_(warmstart0) if warmstart0 is not None else d.qacc_warmstart.zero_()
+    qpos = torch.empty((T, self.nworld, self.nq), dtype=torch.float32, device=d.qpos.device)
+    qvel = torch.empty((T, self.nworld, self.nv), dtype=torch.float32, device=d.qpos.device)
+    for t in range(T):
+      self.s_qpos[t].copy_(d.qpos)
+      self.s_qvel[t].copy_(d.qvel)
+      self.s_act[t].copy_(d.act)
+      self.s_warm[t].copy_(d.qacc_warmstart)
+      d.ctrl.copy_(ctrl[t])
+      api.step_fused(self.m, d)
+      qpos[t].copy_(d.qpos)
+      qvel[t].copy_(d.qvel)
+    return qpos, qvel
+
+  def backward(self, ctrl, qpos, g_qpos, g_qvel):
+    """Gradients (qpos0, qvel0, ctrl) from the gradients with respect to the
+    returned trajectories (the forward pass must have run with ctrl)."""
+    m, d, T = self.m, self.d, self.nstep
+    grad_ctrl = torch.zeros_like(ctrl)
+    # lam_T = g(x_T), at x_T's qpos
+    d.qpos.copy_(qpos[T - 1])
+    adjoint_step(m, d, None, None, self.lam_next, g_qpos=g_qpos[T - 1], g_qvel=g_qvel[T - 1])
+    for t in range(T - 1, -1, -1):
+      self._load(t, ctrl)
+      A, B = self.tfd(d)
+      g_q = g_qpos[t - 1] if t > 0 else None
+      g_v = g_qvel[t - 1] if t > 0 else None
+      adjoint_step(m, d, A, B, self.lam, g_qpos=g_q, g_qvel=g_v, lam_next=self.lam_next, grad_ctrl=grad_ctrl[t])
+      self.lam, self.lam_next = self.lam_next, self.lam
+    # lam_next now holds lam_0; d holds x_0
+    gq0 = torch.empty((self.nworld, self.nq), dtype=torch.float32, device=d.qpos.device)
+    gv0 = torch.empty((self.nworld, self.nv), dtype=torch.float32, device=d.qpos.device)
+    ga0 = torch.empty((self.nworld, self.na), dtype=torch.float32, device=d.qpos.device)
+    adjoint_state_grad(m, d, self.lam_next, gq0, gv0, ga0 if self.na else None)
+    return gq0, gv0, grad_ctrl
+
+  def __call__(self, qpos0, qvel0, ctrl):
+    return _RolloutFn.apply(self, qpos0, qvel0, ctrl)
+
+
+class _RolloutFn(torch.autograd.Function):
+
+  @staticmethod
+  def forward(ctx, r: Rollout, qpos0, qvel0, ctrl):
+    ctrl = ctrl.detach().contiguous()
+    qpos, qvel = r.forward(qpos0.detach(), qvel0.detach(), ctrl)
+    ctx.r = r
+    ctx.save_for_backward(ctrl, qpos)
+    return qpos, qvel
+
+  @staticmethod
+  def backward(ctx, g_qpos, g_qvel):
+    ctrl, qpos = ctx.saved_tensors
+    gq0, gv0, gu = ctx.r.backward(ctrl, qpos, g_qpos.contiguous(), g_qvel.contiguous())
+    return None, gq0, gv0, gu

@@ -9,6 +9,12 @@
                     headline-benchmark overrides: all 21 motor gears 100,
                     implicitfast, Newton, iterations / ls_iterations 1 / 4
                     (the reference benchmark's defaults, testspeed.py:42-59).
+* `humanoid_rollout_state(nworld)` — the differentiable-rollout workload's
+                    initial states: the squat keyframe with the torso
+                    position and the 21 hinge angles perturbed by N(0, 0.05^2),
+                    the torso quaternion rotated by exp of an N(0, 0.05^2)
+                    rotation vector (and normalised), qvel = 0 (numpy
+                    default_rng(3)).
 """
 
 from __future__ import annotations
@@ -63,3 +69,22 @@ def chain_initial_state(nworld: int, seed: int = 0, nq: int = 3):
   qpos = rng.uniform(-0.5, 0.5, (nworld, nq))
   qvel = rng.normal(0.0, 0.1, (nworld, nq))
   return qpos, qvel
+
+
+def humanoid_rollout_state(mjm, nworld: int, seed: int = 3):
+  """(qpos0, qvel0) [nworld, nq] / [nworld, nv] of the differentiable
+  rollout workload (see the module docstring)."""
+  rng = np.random.default_rng(seed)
+  q = np.tile(np.asarray(mjm.key_qpos[0], np.float64), (nworld, 1))
+  q[:, 0:3] += rng.normal(0.0, 0.05, (nworld, 3))
+  q[:, 7:] += rng.normal(0.0, 0.05, (nworld, mjm.nq - 7))
+  v = rng.normal(0.0, 0.05, (nworld, 3))
+  ang = np.linalg.norm(v, axis=1, keepdims=True)
+  dq = np.concatenate([np.cos(0.5 * ang), v * (np.sin(0.5 * ang) / np.maximum(ang, 1e-300))], 1)
+  a, b = q[:, 3:7], dq
+  q[:, 3:7] = np.stack([a[:, 0] * b[:, 0] - a[:, 1] * b[:, 1] - a[:, 2] * b[:, 2] - a[:, 3] * b[:, 3],
+                        a[:, 0] * b[:, 1] + a[:, 1] * b[:, 0] + a[:, 2] * b[:, 3] - a[:, 3] * b[:, 2],
+                        a[:, 0] * b[:, 2] - a[:, 1] * b[:, 3] + a[:, 2] * b[:, 0] + a[:, 3] * b[:, 1],
+                        a[:, 0] * b[:, 3] + a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1] + a[:, 3] * b[:, 0]], 1)
+  q[:, 3:7] /= np.linalg.norm(q[:, 3:7], axis=1, keepdims=True)
+  return q, np.zeros((nworld, mjm.nv))

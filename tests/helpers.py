@@ -91,3 +91,30 @@ def pairs_of(hd, world=0):
   adr, n = hd["world_pair"].reshape(-1, 2)[world]
   p = hd["collision_pair"].reshape(-1, 2)[adr:adr + n]
   return sorted(tuple(sorted((int(a), int(b)))) for a, b in p)
+
+
+def oracle_stepper(mjm, m_cpu, real=np.float64, nconmax_w=64, njmax_w=277):
+  """step(qpos, qvel, act, ctrl, warm) -> (qpos', qvel', act', warm') over
+  [n, .] arrays, one oracle step of n worlds (host data cached per n)."""
+  hm = host_model_exact(mjm, m_cpu, real) if real == np.float64 else host_model(m_cpu, real)
+  cache = {}
+
+  def step(q, v, a, u, w):
+    n = q.shape[0]
+    if n not in cache:
+      d = io.make_data(mjm, nworld=n, nconmax=nconmax_w * n, njmax=njmax_w * n, device="cpu")
+      cache[n] = host_data(m_cpu, d, real)
+    hd = cache[n]
+    hd["qpos"][:] = q
+    hd["qvel"][:] = v
+    if mjm.na:
+      hd["act"][:] = a
+    if mjm.nu:
+      hd["ctrl"][:] = u
+    hd["qacc_warmstart"][:] = w
+    oracle.run("step", hm, hd)
+    act = hd["act"].copy() if mjm.na else np.zeros((n, 0))
+    return (hd["qpos"].astype(np.float64), hd["qvel"].astype(np.float64), act.astype(np.float64),
+            hd["qacc_warmstart"].astype(np.float64))
+
+  return step
