@@ -33,6 +33,7 @@ from .api import step_fused as step_fused
 from .api import StepGraph as StepGraph
 from .api import transmission as transmission
 from .api import xfrc_accumulate as xfrc_accumulate
+from . import derivative as derivative
 from .bridge import FunctionalStep as FunctionalStep
 from .bridge import Unroll as Unroll
 from .io import get_data_into as get_data_into

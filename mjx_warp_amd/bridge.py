@@ -15,8 +15,9 @@ which XLA records as one CUDA graph (jax_unroll.py:63-79).  Here:
   graph between static input and output buffers; each call copies the
   inputs in, replays the graph and returns copies of the outputs.
 
-The step has no adjoint here (neither does the reference's), so these are
-forward-only: inputs that require gradients are rejected.
+These are forward-only (inputs that require gradients are rejected); the
+differentiable rollout, with an adjoint the reference does not have, is
+`mjx_warp_amd.derivative.Rollout`.
 """
 
 from __future__ import annotations
@@ -35,7 +36,8 @@ _KEYS = itertools.count()
 def _check(*tensors):
   for t in tensors:
     if t is not None and t.requires_grad:
-      raise RuntimeError("the step has no adjoint: its inputs cannot require gradients")
+      raise RuntimeError("FunctionalStep / Unroll are forward-only: use mjx_warp_amd.derivative.Rollout "
+                         "for gradients")
 
 
 class FunctionalStep:
