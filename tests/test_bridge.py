@@ -34,7 +34,7 @@ def test_inputs_requiring_grad_are_rejected():
   d = mjw.make_data(mjm, nworld=2, device="cpu")
   f = mjw.FunctionalStep(m, d)
   q = d.qpos.clone().requires_grad_(True)
-  with pytest.raises(RuntimeError, match="adjoint"):
+  with pytest.raises(RuntimeError, match="forward-only"):
     f(q, d.qvel)
   f.close()
 
