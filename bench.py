@@ -139,22 +139,34 @@ def time_form(args, fused: bool):
 
 
 def time_rollout_graph(args, nstep: int = 10):
-  """The batched-persistent form: nstep fused steps captured once as a HIP
-  graph and replayed (ctrl resampled between replays)."""
+  """The batched-persistent form: nstep fused steps, each after copying its
+  own ctrl row into d.ctrl, captured once as a HIP graph and replayed; the
+  nstep ctrl rows are resampled from U(-1, 1) before each replay, so every
+  step sees fresh controls as in the eager loop."""
   import torch
 
   import mjx_warp_amd as mjw
+  from mjx_warp_amd import _native
 
   _, m, d = setup(args)
   gen = torch.Generator(device="cuda")
   gen.manual_seed(1)
-  graph = mjw.StepGraph(m, d, nstep=nstep, fused=True)
-  u = torch.empty_like(d.ctrl)
+  u = torch.empty((nstep,) + tuple(d.ctrl.shape), dtype=torch.float32, device="cuda")
+  _native.check(m, d)
+  object.__setattr__(d, "_checked", m)
+  graph = torch.cuda.CUDAGraph()
+  side = torch.cuda.Stream()
+  side.wait_stream(torch.cuda.current_stream())
+  with torch.cuda.graph(graph, stream=side):
+    for k in range(nstep):
+      d.ctrl.copy_(u[k])
+      _native.call("mjh_step_fused", m, d)
+  torch.cuda.current_stream().wait_stream(side)
   reps = max(1, args.steps // nstep)
 
   def one():
     torch.rand(u.shape, generator=gen, device="cuda", out=u)
-    d.ctrl.copy_(u.mul_(2.0).sub_(1.0))
+    u.mul_(2.0).sub_(1.0)
     graph.replay()
 
   for _ in range(max(1, args.warmup // nstep)):
