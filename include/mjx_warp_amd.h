@@ -456,6 +456,24 @@ int mjh_step_fused(const mjhModel* m, const mjhData* d, mjhStream stream);
 /* forward.py:43-162 and 682-695 for nstep steps in one call */
 int mjh_rollout(const mjhModel* m, const mjhData* d, int nstep, int fused, mjhStream stream);
 
+/* ---- heterogeneous batches ----
+ * No counterpart in the reference, whose batch is one Model (types.py: every
+ * world of a Data shares the Model).  A plan groups several (model, data)
+ * pairs -- each group its own model and its own worlds -- so that one fused
+ * step advances all of them with one launch per stage over every group
+ * (workgroups find their group in a device table), instead of one step call
+ * per model.  Groups with nv <= 32 run on 32-lane teams, larger ones on
+ * 64-lane teams; each group's Data keeps its own pools and counters. */
+typedef struct mjhMulti mjhMulti;
+/* Checks every group as mjh_check does and builds the plan (the one call
+ * here that allocates: the group table in device memory).  The plan holds
+ * the groups' pointers: their buffers must outlive it. */
+int mjh_multi_create(int ngroups, const mjhModel* models, const mjhData* datas, mjhMulti** out);
+/* One fused step (mjh_step_fused) of every world of every group. */
+int mjh_multi_step(const mjhMulti* plan, mjhStream stream);
+/* Frees the plan (after its last step has finished). */
+int mjh_multi_destroy(mjhMulti* plan);
+
 /* ---- transition derivatives and the adjoint ----
  * The reference has no gradients (its kernels overwrite arrays in place;
  * collision_convex.py:22 disables backward code generation); these follow

@@ -35,6 +35,7 @@ from .api import transmission as transmission
 from .api import xfrc_accumulate as xfrc_accumulate
 from . import derivative as derivative
 from .bridge import FunctionalStep as FunctionalStep
+from .multi import MultiStep as MultiStep
 from .bridge import Unroll as Unroll
 from .io import get_data_into as get_data_into
 from .tracing import EventTracer as EventTracer

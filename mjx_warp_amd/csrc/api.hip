@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <vector>
 
 #include "collision.hpp"
 #include "constraint.hpp"
@@ -223,13 +224,16 @@ __global__ void __launch_bounds__(64) k_stage(mjhModel m, mjhData d, int lds_per
 
 // counter reset without memset nodes (capture-safe):
 // which bit 0: ncollision + ncon, bit 1: nefc
+MJH_DEV void reset_counters(const mjhData& d, int which) {
+  if (which & 1) {
+    d.ncollision[0] = 0;
+    d.ncon[0] = 0;
+  }
+  if (which & 2) d.nefc[0] = 0;
+}
 __global__ void k_reset(mjhData d, int which) {
   if (threadIdx.x == 0) {
-    if (which & 1) {
-      d.ncollision[0] = 0;
-      d.ncon[0] = 0;
-    }
-    if (which & 2) d.nefc[0] = 0;
+    reset_counters(d, which);
   }
 }
 
@@ -532,6 +536,96 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
   return launch_team<StTail<true>, kTeam>(m, d, s, lds_linalg(m), "step_tail");
 }
 
+// ---- heterogeneous batches: several (model, data) groups, each with its
+// own model and worlds, advanced by one launch per stage over all groups ----
+
+// One group as a launch sees it: the group's workgroups are
+// [first_block, first_block + its block count) of the grid.
+struct GroupDesc {
+  mjhModel m;
+  mjhData d;
+  int first_block;
+  Tune tune[2];  // [0] the solve / tail launches, [1] the big-world launch
+};
+
+// The group of workgroup b: the last with first_block <= b (uniform per
+// workgroup; first_block ascends and the first is 0).
+MJH_DEV const GroupDesc& group_of(const GroupDesc* __restrict__ g, int ngroups, int b) {
+  int lo = 0, hi = ngroups - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g[mid].first_block <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return g[lo];
+}
+
+template <int TEAM, class S, int TI>
+__global__ void __launch_bounds__(64) k_stage_groups(const GroupDesc* __restrict__ g, int ngroups, int lds_per_world) {
+  extern __shared__ float smem[];
+  const int b = blockIdx.x;
+  const GroupDesc& G = group_of(g, ngroups, b);
+  Team<TEAM> t(G.d.nworld, b - G.first_block);
+  S()(G.m, G.d, smem + t.slot * lds_per_world, t, G.tune[TI]);
+}
+
+// one workgroup per group
+__global__ void k_reset_groups(const GroupDesc* __restrict__ g) {
+  if (threadIdx.x != 0) return;
+  const GroupDesc& G = g[blockIdx.x];
+  reset_counters(G.d, (G.m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3);
+}
+
+// A device table of groups for launches of one team size.
+struct GroupTable {
+  GroupDesc* dev = nullptr;
+  int ngroups = 0;
+  int blocks = 0;
+};
+
+}  // namespace
+
+// The plan of a heterogeneous batch: device tables of its groups (32-lane
+// groups: nv <= 32, seen by the 32-lane launches and by the 64-lane tail;
+// 64-lane groups) and each launch's per-world LDS stride.
+struct mjhMulti {
+  GroupTable t32, t32_64, t64;
+  int lds_fwd = 0, lds_solve = 0, lds_big = 0, lds_tail32 = 0, lds_step = 0, lds_tail64 = 0;
+};
+
+namespace {
+
+int build_table(const std::vector<GroupDesc>& host, int wpb, GroupTable* t) {
+  t->ngroups = (int)host.size();
+  if (host.empty()) return MJH_OK;
+  std::vector<GroupDesc> h = host;
+  long long blocks = 0;
+  for (GroupDesc& g : h) {
+    g.first_block = (int)blocks;
+    blocks += (g.d.nworld + wpb - 1) / wpb;
+  }
+  if (blocks > 0x7fffffffLL) return fail(MJH_ERR_CAPACITY, "multi: too many worlds");
+  t->blocks = (int)blocks;
+  TRY(check_hip(hipMalloc(&t->dev, h.size() * sizeof(GroupDesc)), "multi: table"));
+  return check_hip(hipMemcpy(t->dev, h.data(), h.size() * sizeof(GroupDesc), hipMemcpyHostToDevice), "multi: table");
+}
+
+void free_tables(mjhMulti* p) {
+  for (GroupTable* t : {&p->t32, &p->t32_64, &p->t64})
+    if (t->dev) (void)hipFree(t->dev);
+}
+
+template <int TEAM, class S, int TI>
+int launch_groups(const GroupTable& t, int ldsw, hipStream_t s, const char* name) {
+  if (t.ngroups == 0 || t.blocks == 0) return MJH_OK;
+  constexpr int wpb = 64 / TEAM;
+  const size_t bytes = (size_t)ldsw * wpb * sizeof(float);
+  hipLaunchKernelGGL((k_stage_groups<TEAM, S, TI>), dim3(t.blocks), dim3(64), bytes, s, t.dev, t.ngroups, ldsw);
+  return check_launch(name);
+}
+
+int max_of(int a, int b) { return a > b ? a : b; }
+
 }  // namespace
 
 extern "C" {
@@ -643,6 +737,87 @@ int mjh_xfrc_accumulate(const mjhModel* m, const mjhData* d, float* qfrc, mjhStr
   if (d->nworld <= 0) return MJH_OK;
   hipLaunchKernelGGL(k_xfrc, dim3(d->nworld), dim3(64), 0, (hipStream_t)stream, *m, *d, qfrc);
   return check_launch("xfrc_accumulate");
+}
+
+int mjh_multi_create(int ngroups, const mjhModel* models, const mjhData* datas, mjhMulti** out) {
+  if (!out) return fail(MJH_ERR_INVALID, "multi: null output");
+  *out = nullptr;
+  if (ngroups <= 0 || !models || !datas) return fail(MJH_ERR_INVALID, "multi: no groups");
+  std::vector<GroupDesc> g32, g64;
+  mjhMulti plan;
+  for (int i = 0; i < ngroups; i++) {
+    const mjhModel& m = models[i];
+    const mjhData& d = datas[i];
+    const std::string at = "multi: group " + std::to_string(i) + ": ";
+    if (mjh_check(&m, &d) != MJH_OK) return fail(MJH_ERR_INVALID, at + g_last_error);
+    if (m.opt.integrator != mjhINT_EULER && m.opt.integrator != mjhINT_IMPLICITFAST)
+      return fail(MJH_ERR_UNSUPPORTED, at + "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
+    TRY(check_implicit_dense(m));
+    GroupDesc gd{};
+    gd.m = m;
+    gd.d = d;
+    gd.tune[0] = tune_for(m);
+    if (reg_path(m)) {
+      gd.tune[1] = tune_tail_reg(m);
+      g32.push_back(gd);
+      plan.lds_fwd = max_of(plan.lds_fwd, lds_forward(m));
+      plan.lds_solve = max_of(plan.lds_solve, lds_solve_reg(m));
+      plan.lds_big = max_of(plan.lds_big, lds_tail_reg(m));
+      plan.lds_tail32 = max_of(plan.lds_tail32, lds_linalg(m));
+    } else {
+      gd.tune[1] = gd.tune[0];
+      g64.push_back(gd);
+      plan.lds_step = max_of(plan.lds_step, lds_step(m));
+      plan.lds_tail64 = max_of(plan.lds_tail64, lds_linalg(m));
+    }
+  }
+  // each launch's LDS: two worlds per workgroup on 32-lane teams, one on 64
+  const int need = max_of(max_of(2 * plan.lds_fwd, 2 * plan.lds_solve), max_of(2 * plan.lds_big, plan.lds_tail32));
+  if ((size_t)max_of(need, max_of(plan.lds_step, plan.lds_tail64)) * sizeof(float) > 65536)
+    return fail(MJH_ERR_CAPACITY, "multi: a group's model is too large for the dense LDS kernels");
+  mjhMulti* p = new mjhMulti(plan);
+  int rc = build_table(g32, 2, &p->t32);
+  if (rc == MJH_OK) rc = build_table(g32, 1, &p->t32_64);
+  if (rc == MJH_OK) rc = build_table(g64, 1, &p->t64);
+  if (rc != MJH_OK) {
+    free_tables(p);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return MJH_OK;
+}
+int mjh_multi_step(const mjhMulti* p, mjhStream stream) {
+  if (!p) return fail(MJH_ERR_INVALID, "multi: null plan");
+  hipStream_t s = (hipStream_t)stream;
+  for (const GroupTable* t : {&p->t32, &p->t64}) {
+    if (t->ngroups == 0) continue;
+    hipLaunchKernelGGL(k_reset_groups, dim3(t->ngroups), dim3(64), 0, s, t->dev);
+    TRY(check_launch("multi: reset"));
+  }
+  if (p->t32.ngroups) {
+    TRY((launch_groups<kRegTeam, StFusedForward, 0>(p->t32, p->lds_fwd, s, "multi: forward")));
+    TRY(beside(
+        s,
+        [&](hipStream_t q) {
+          return launch_groups<kRegTeam, StBigWorlds<true>, 1>(p->t32, p->lds_big, q, "multi: big worlds");
+        },
+        [&](hipStream_t q) {
+          return launch_groups<kRegTeam, StFusedSolveIntegrate, 0>(p->t32, p->lds_solve, q, "multi: solve");
+        }));
+    TRY((launch_groups<kTeam, StTail<true>, 0>(p->t32_64, p->lds_tail32, s, "multi: tail")));
+  }
+  if (p->t64.ngroups) {
+    TRY((launch_groups<kTeam, StFusedStep, 0>(p->t64, p->lds_step, s, "multi: step")));
+    TRY((launch_groups<kTeam, StTail<true>, 0>(p->t64, p->lds_tail64, s, "multi: tail")));
+  }
+  return MJH_OK;
+}
+int mjh_multi_destroy(mjhMulti* p) {
+  if (!p) return MJH_OK;
+  free_tables(p);
+  delete p;
+  return MJH_OK;
 }
 
 int mjh_transition_fd(const mjhModel* m, const mjhData* d, const mjhData* fd, float eps, int centered, float* A,

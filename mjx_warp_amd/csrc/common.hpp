@@ -223,10 +223,12 @@ struct Team {
   int slot;   // world slot inside the workgroup
   int world;  // global world index (may be >= nworld for padding slots)
   bool valid;
-  MJH_DEV Team(int nworld) {
+  MJH_DEV Team(int nworld) : Team(nworld, blockIdx.x) {}
+  // block: the workgroup's index among those covering these nworld worlds
+  MJH_DEV Team(int nworld, int block) {
     lane = threadIdx.x % TEAM;
     slot = threadIdx.x / TEAM;
-    world = blockIdx.x * wpb + slot;
+    world = block * wpb + slot;
     valid = world < nworld;
   }
 };

@@ -15,6 +15,12 @@
                     the torso quaternion rotated by exp of an N(0, 0.05^2)
                     rotation vector (and normalised), qvel = 0 (numpy
                     default_rng(3)).
+* `box_world(links, radii)` / `heterogeneous_groups(...)` — the
+                    heterogeneous workload's models: a hinge chain of
+                    `links` 0.1 m capsule links hung from (0, 0, 0.95) plus
+                    free spheres of the given radii (density 1000) in a
+                    1 m box (floor and four walls at x, y = +-0.5); one model
+                    per (links, spheres) pair, worlds split over the pairs.
 """
 
 from __future__ import annotations
@@ -88,3 +94,77 @@ def humanoid_rollout_state(mjm, nworld: int, seed: int = 3):
                         a[:, 0] * b[:, 3] + a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1] + a[:, 3] * b[:, 0]], 1)
   q[:, 3:7] /= np.linalg.norm(q[:, 3:7], axis=1, keepdims=True)
   return q, np.zeros((nworld, mjm.nv))
+
+
+def box_world_xml(links: int, radii) -> str:
+  """A chain of `links` hinge links (0.1 m capsules of radius 0.02, 0.1 kg,
+  hinges about y limited to +-1.57) hung from (0, 0, 0.95), and one free
+  sphere per radius (density 1000), inside a 1 m box: a floor and four walls
+  (planes at x, y = +-0.5 facing inward).  Newton, implicitfast,
+  dt 0.002."""
+  body = ""
+  for i in reversed(range(links)):
+    pos = "0 0 0.95" if i == 0 else "0 0 -0.1"
+    body = (f'<body name="link{i}" pos="{pos}"><joint type="hinge" axis="0 1 0" range="-1.57 1.57" '
+            f'limited="true"/><geom type="capsule" fromto="0 0 0 0 0 -0.1" size="0.02" mass="0.1"/>'
+            + body + "</body>")
+  spheres = "".join(f'<body name="ball{k}" pos="0 0 0.5"><freejoint/><geom type="sphere" size="{r:.6g}" '
+                    f'density="1000"/></body>' for k, r in enumerate(radii))
+  return f"""
+<mujoco model="box{links}x{len(radii)}">
+  <compiler angle="radian"/>
+  <option timestep="0.002" integrator="implicitfast" solver="Newton" iterations="20" ls_iterations="10"/>
+  <worldbody>
+    <geom name="floor" type="plane" size="0 0 0.05"/>
+    <geom name="wall_px" type="plane" pos="0.5 0 0.5" zaxis="-1 0 0" size="0.5 0.5 0.05"/>
+    <geom name="wall_nx" type="plane" pos="-0.5 0 0.5" zaxis="1 0 0" size="0.5 0.5 0.05"/>
+    <geom name="wall_py" type="plane" pos="0 0.5 0.5" zaxis="0 -1 0" size="0.5 0.5 0.05"/>
+    <geom name="wall_ny" type="plane" pos="0 -0.5 0.5" zaxis="0 1 0" size="0.5 0.5 0.05"/>
+    {body}
+    {spheres}
+  </worldbody>
+</mujoco>"""
+
+
+def box_world(links: int, radii) -> mjcf.MjModel:
+  return mjcf.from_xml_string(box_world_xml(links, radii))
+
+
+def box_world_states(mjm, links: int, radii, nworld: int, rng):
+  """(qpos, qvel) [nworld, nq] / [nworld, nv]: the chain hanging straight,
+  each sphere at rest at a uniform position inside the box (clear of the
+  walls, the floor, the chain's column |x|, |y| < 0.1 + r, and the other
+  spheres; rejection sampling), identity orientation."""
+  q = np.zeros((nworld, mjm.nq))
+  for w in range(nworld):
+    placed = []
+    for k, r in enumerate(radii):
+      for _ in range(1000):
+        p = rng.uniform([-0.5 + r, -0.5 + r, r], [0.5 - r, 0.5 - r, 0.9 - r])
+        if abs(p[0]) < 0.1 + r and abs(p[1]) < 0.1 + r:
+          continue
+        if all(np.linalg.norm(p - c) > r + rc for c, rc in placed):
+          break
+      placed.append((p, r))
+      a = links + 7 * k
+      q[w, a:a + 3] = p
+      q[w, a + 3] = 1.0
+  return q, np.zeros((nworld, mjm.nv))
+
+
+def heterogeneous_groups(nworld: int = 16384, seed: int = 2, max_links: int = 8, max_spheres: int = 4):
+  """Worlds of the heterogeneous workload split by (links, spheres): links
+  ~ U{2..max_links}, spheres ~ U{1..max_spheres} per world (numpy
+  default_rng(seed)); each pair's sphere radii ~ U(0.03, 0.1), drawn once
+  per pair.  Returns [(links, radii, nworld of the pair)] sorted by pair.
+  With the defaults every model has nv = links + 6 spheres <= 32."""
+  rng = np.random.default_rng(seed)
+  L = rng.integers(2, max_links + 1, nworld)
+  S = rng.integers(1, max_spheres + 1, nworld)
+  out = []
+  for links in range(2, max_links + 1):
+    for s in range(1, max_spheres + 1):
+      n = int(np.sum((L == links) & (S == s)))
+      if n:
+        out.append((links, tuple(float(r) for r in rng.uniform(0.03, 0.1, s)), n))
+  return out

@@ -1,0 +1,131 @@
+"""Heterogeneous batches: several models, each with its own worlds, stepped
+together.
+
+The reference batches worlds of one Model (types.py: every world of a Data
+shares it).  `MultiStep([(m0, d0), (m1, d1), ...])` takes groups of
+(Model, Data) -- a different model per group, any number of worlds each --
+and advances every world of every group by one fused step per `step()`.
+In the "grouped" mode the library runs one launch per stage over all groups
+(`mjh_multi_step`: each workgroup finds its group in a device table), so
+many small groups cost the launches of one batch; in the "per_group" mode
+it issues one `mjh_step_fused` per group.  `autotune=True` times both modes
+on the groups' own state (restored afterwards) and keeps the faster one,
+remembered per group signature (each group's sizes and world count) for
+the rest of the process.
+
+The plan holds the groups' device pointers: write Data fields in place
+(`d.qpos.copy_(...)`); a field replaced by a new tensor is picked up by
+rebuilding the plan on the next `step()`.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+
+from . import _native
+from .types import Data, Model
+
+_TUNED: dict = {}
+
+_STATE = ("qpos", "qvel", "act", "ctrl", "qacc_warmstart")
+
+
+def _signature(groups) -> tuple:
+  return tuple((int(m.nq), int(m.nv), int(m.nbody), int(m.ngeom), int(m.nu), int(d.nworld), int(d.njmax))
+               for m, d in groups)
+
+
+class MultiStep:
+  """One fused step of every world of every (Model, Data) group per call."""
+
+  def __init__(self, groups, autotune: bool = False, mode: str = "grouped"):
+    if not groups:
+      raise ValueError("MultiStep needs at least one (Model, Data) group")
+    if mode not in ("grouped", "per_group"):
+      raise ValueError(f"unknown mode {mode!r}")
+    self.groups = [(m, d) for m, d in groups]
+    self.mode = mode
+    self._plan = None
+    self._structs = None
+    self._build()
+    if autotune:
+      key = _signature(self.groups)
+      if key not in _TUNED:
+        _TUNED[key] = self._tune()
+      self.mode = _TUNED[key]
+
+  def _build(self):
+    lib = _native.lib()
+    self.close()
+    ms = [_native.model_struct(m) for m, _ in self.groups]
+    ds = [_native.data_struct(m, d) for m, d in self.groups]
+    n = len(self.groups)
+    M = (type(ms[0]) * n)(*ms)
+    D = (type(ds[0]) * n)(*ds)
+    plan = ctypes.c_void_p()
+    rc = lib.mjh_multi_create(n, M, D, ctypes.byref(plan))
+    if rc != _native.MJH_OK:
+      _native._raise(rc, "mjh_multi_create")
+    self._plan = plan
+    self._structs = ds
+
+  def _stale(self) -> bool:
+    return any(d._abi is not s for (_, d), s in zip(self.groups, self._structs))
+
+  def step(self):
+    """Advances every world of every group by one step (asynchronous on the
+    current stream; each group's `d.time` advances on the host)."""
+    if self._stale():
+      self._build()
+    if self.mode == "grouped":
+      rc = _native.lib().mjh_multi_step(self._plan, _native._stream())
+      if rc != _native.MJH_OK:
+        _native._raise(rc, "mjh_multi_step")
+    else:
+      for m, d in self.groups:
+        _native.call("mjh_step_fused", m, d)
+    for m, d in self.groups:
+      d.time = d.time + m.opt.timestep
+
+  def _tune(self, steps: int = 5) -> str:
+    import torch
+
+    saved = [({f: getattr(d, f).clone() for f in _STATE}, d.time) for _, d in self.groups]
+
+    def restore():
+      for (_, d), (fields, t) in zip(self.groups, saved):
+        for f, v in fields.items():
+          getattr(d, f).copy_(v)
+        d.time = t
+
+    best, best_t = self.mode, float("inf")
+    for mode in ("grouped", "per_group"):
+      self.mode = mode
+      self.step()  # warm-up
+      torch.cuda.synchronize()
+      t0 = time.perf_counter()
+      for _ in range(steps):
+        self.step()
+      torch.cuda.synchronize()
+      dt = time.perf_counter() - t0
+      restore()
+      if dt < best_t:
+        best, best_t = mode, dt
+    torch.cuda.synchronize()
+    return best
+
+  def close(self):
+    """Frees the plan (synchronises the device first)."""
+    if self._plan is not None and self._plan.value:
+      import torch
+
+      torch.cuda.synchronize()
+      _native.lib().mjh_multi_destroy(self._plan)
+    self._plan = None
+
+  def __del__(self):
+    try:
+      self.close()
+    except Exception:
+      pass
