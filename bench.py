@@ -15,8 +15,9 @@ all state resident in HBM; the fused step (a forward launch and a
 solve+integrate launch, plus a tail for worlds over the solver's row
 capacity) is the headline, the plain per-stage step is reported beside it,
 and so are the batched-persistent form (10 fused steps captured as one HIP
-graph, replayed) and forward + adjoint world-steps/s of a differentiable
-50-step rollout of 1024 humanoids (`forward_adjoint`).
+graph, replayed), forward + adjoint world-steps/s of a differentiable
+50-step rollout of 1024 humanoids (`forward_adjoint`), and a heterogeneous
+batch of 16,384 worlds over 28 different models (`heterogeneous`).
 `roofline.traffic` is the fused step's HBM traffic measured with rocprofv3
 for the default configuration (profiles/step_humanoid_b8192_newton10.md).
 
@@ -238,6 +239,66 @@ def time_adjoint(args, nworld: int = 1024, nstep: int = 50, eps: float = 1e-3, r
                       "Jacobians (eps 1e-3) + adjoint recursion"}
 
 
+def time_heterogeneous(args, nworld: int = 16384, steps: int = 50, graph_steps: int = 10):
+  """World-steps/s of the heterogeneous workload: 16,384 worlds, each a
+  hinge chain of 2-8 links plus 1-4 free spheres in a walled box, split
+  into one model per (links, spheres) pair (28 models, nv 8-32), all
+  advanced together: grouped (one launch per stage over every model),
+  per model (one fused step per model), and the grouped step captured as a
+  HIP graph of graph_steps steps and replayed."""
+  import numpy as np
+  import torch
+
+  import mjx_warp_amd as mjw
+  from mjx_warp_amd import models
+
+  rng = np.random.default_rng(2)
+  groups = []
+  spec = models.heterogeneous_groups(nworld)
+  for links, radii, n in spec:
+    mjm = models.box_world(links, radii)
+    q, v = models.box_world_states(mjm, links, radii, n, rng)
+    m = mjw.put_model(mjm)
+    ncon = 8 * (links + len(radii))
+    d = mjw.make_data(mjm, nworld=n, nconmax=ncon * n, njmax=(4 * ncon + mjm.njnt) * n)
+    d.qpos.copy_(torch.tensor(q, dtype=torch.float32))
+    groups.append((m, d))
+  ms = mjw.MultiStep(groups)
+  out = {"nworld": nworld, "models": len(groups), "steps": steps,
+         "nv_range": [min(int(m.nv) for m, _ in groups), max(int(m.nv) for m, _ in groups)]}
+  for mode in ("grouped", "per_group"):
+    ms.mode = mode
+    for _ in range(3):
+      ms.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+      ms.step()
+    torch.cuda.synchronize()
+    out[mode] = nworld * steps / (time.perf_counter() - t0)
+  ms.mode = "grouped"
+  g = torch.cuda.CUDAGraph()
+  side = torch.cuda.Stream()
+  side.wait_stream(torch.cuda.current_stream())
+  with torch.cuda.graph(g, stream=side):
+    for _ in range(graph_steps):
+      ms.step()
+  torch.cuda.current_stream().wait_stream(side)
+  g.replay()
+  torch.cuda.synchronize()
+  reps = max(1, steps // graph_steps)
+  t0 = time.perf_counter()
+  for _ in range(reps):
+    g.replay()
+  torch.cuda.synchronize()
+  out["grouped_graph"] = nworld * reps * graph_steps / (time.perf_counter() - t0)
+  out["value"] = max(out["grouped"], out["per_group"], out["grouped_graph"])
+  out["unit"] = "world-steps/s"
+  out["finite"] = bool(all(torch.isfinite(d.qpos).all().item() for _, d in groups))
+  ms.close()
+  return out
+
+
 def cpu_baseline(args, target_s: float = 15.0):
   """The oracle (FP32 CPU restatement, OpenMP over worlds) on this host."""
   import numpy as np
@@ -306,6 +367,7 @@ def main():
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
   ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint rollout")
   ap.add_argument("--no-rollout-graph", action="store_true", help="skip the graph-replayed rollout")
+  ap.add_argument("--no-hetero", action="store_true", help="skip the heterogeneous (many-model) batch")
   ap.add_argument("--graph", action="store_true", help="replay a HIP graph of one captured step")
   ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                   help="only run a plain copy of MIB MiB (counter calibration) and exit")
@@ -333,6 +395,11 @@ def main():
   if not args.no_rollout_graph:
     rgraph = time_rollout_graph(args)
     log(f"rollout graph: {rgraph['value']:.4g} world-steps/s")
+  het = None
+  if not args.no_hetero:
+    het = time_heterogeneous(args)
+    log(f"heterogeneous: grouped {het['grouped']:.4g}, per model {het['per_group']:.4g}, "
+        f"grouped graph {het['grouped_graph']:.4g} world-steps/s")
   adj = None
   if not args.no_adjoint:
     adj = time_adjoint(args)
@@ -375,6 +442,7 @@ def main():
                                               "kernel_ms": plain["kernel_ms"]},
     "rollout_graph": rgraph,
     "forward_adjoint": adj,
+    "heterogeneous": het,
   }
   print(json.dumps(out), flush=True)
 
