@@ -5,7 +5,8 @@
  * model description, a data description and a HIP stream.  Both
  * descriptions are plain structs of device pointers and sizes; the caller
  * (normally the Python package over ctypes) owns every buffer, the library
- * allocates no memory inside a call and never synchronises the stream.  It
+ * allocates no memory inside a call (mjh_multi_create, which builds a plan,
+ * aside) and never synchronises the stream.  It
  * creates, once per host thread and device, a second stream and two events
  * that it keeps until the process ends (see mjh_solve).
  *
