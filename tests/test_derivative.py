@@ -194,11 +194,12 @@ def test_transition_fd_humanoid_matches_oracle():
     A, B = A.cpu().numpy().astype(np.float64), B.cpu().numpy().astype(np.float64)
     Ar, Br = od.transition_fd(step, mjm, q, v, np.zeros((n, 0)), u, w0, 1e-3)
     ea, eb = _jac_errors(A, B, Ar, Br)
-    print(f"state {t}: |dA|/|A| {np.array2string(ea, precision=1)}  |dB|/|B| {np.array2string(eb, precision=1)}")
+    fmt = {"float_kind": lambda x: f"{x:.1e}"}
+    print(f"state {t}: |dA|/|A| {np.array2string(ea, formatter=fmt)}  |dB|/|B| {np.array2string(eb, formatter=fmt)}")
     assert np.all(ea < 1e-2) and np.all(eb < 1e-2), (t, ea, eb)
 
 
-def _gpu_rollout_grads(mjm, q0, v0, ctrl, eps=1e-3):
+def _gpu_rollout_grads(mjm, q0, v0, ctrl, eps=4e-3):
   n, T = q0.shape[0], ctrl.shape[0]
   m = mjw.put_model(mjm)
   r = derivative.Rollout(mjm, m, n, T, nconmax=64 * n, njmax=277 * n, eps=eps)
@@ -219,9 +220,9 @@ def _rel(a, b):
 @pytest.mark.gpu
 def test_rollout_gradient_smooth_matches_oracle_fd():
   """Contacts and joint limits disabled (smooth dynamics: actuators, gravity,
-  Coriolis, implicitfast): the device adjoint's gradients with respect to
-  ctrl and qvel0 against central differences (eps 1e-6) of whole rollouts
-  of the FP64 oracle."""
+  Coriolis, implicitfast): the device adjoint's gradients (FP32, eps 4e-3)
+  with respect to ctrl and qvel0 against central differences (eps 1e-6) of
+  whole rollouts of the FP64 oracle, to 1e-3 relative error."""
   mjm = _humanoid()
   mjm.opt.disableflags |= int(mjw.DisableBit.CONTACT) | int(mjw.DisableBit.LIMIT)
   n, T = 4, 5
@@ -232,13 +233,13 @@ def test_rollout_gradient_smooth_matches_oracle_fd():
   fd_u, fd_v0 = od.loss_grad_fd(step, _loss, q0, v0, z0, ctrl, w0, 1e-6)
   eu, ev = _rel(gu, fd_u), _rel(gv0, fd_v0)
   print(f"smooth: ctrl gradient rel. error {eu:.2e}, qvel0 gradient rel. error {ev:.2e}")
-  assert eu < 1e-2 and ev < 1e-2, (eu, ev)
+  assert eu < 1e-3 and ev < 1e-3, (eu, ev)
 
 
 @pytest.mark.gpu
 def test_rollout_gradient_in_contact_matches_oracle_adjoint():
   """Humanoids in contact with the floor: the device adjoint's gradients
-  against the oracle's adjoint over FP64 Jacobians at the same eps (1e-3);
+  against the oracle's adjoint over FP64 Jacobians at the same eps (4e-3);
   the distance to the eps 1e-6 loss differences, which columns straddling a
   contact or limit switch inside +-eps set, is printed, not held."""
   mjm = _humanoid()
@@ -248,12 +249,12 @@ def test_rollout_gradient_in_contact_matches_oracle_adjoint():
   step = helpers.oracle_stepper(mjm, mjw.put_model(mjm, device="cpu"))
   z0, w0 = np.zeros((n, 0)), np.zeros((n, mjm.nv))
   traj, inputs = od.rollout(step, q0, v0, z0, ctrl, w0)
-  jac = [od.transition_fd(step, mjm, *inp, 1e-3) for inp in inputs]
+  jac = [od.transition_fd(step, mjm, *inp, 4e-3) for inp in inputs]
   lam0, gu_r = od.adjoint(mjm, jac, inputs, traj, *_loss_grads(traj))
   gu_r = gu_r + 2e-3 * ctrl
   gv_r = lam0[:, mjm.nv:2 * mjm.nv]
   fd_u, fd_v0 = od.loss_grad_fd(step, _loss, q0, v0, z0, ctrl, w0, 1e-6)
   eu, ev = _rel(gu, gu_r), _rel(gv0, gv_r)
-  print(f"in contact: vs oracle adjoint (eps 1e-3) ctrl {eu:.2e} qvel0 {ev:.2e}; "
+  print(f"in contact: vs oracle adjoint (eps 4e-3) ctrl {eu:.2e} qvel0 {ev:.2e}; "
         f"vs eps 1e-6 loss differences ctrl {_rel(gu, fd_u):.2e} qvel0 {_rel(gv0, fd_v0):.2e}")
   assert eu < 5e-2 and ev < 5e-2, (eu, ev)
