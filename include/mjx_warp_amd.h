@@ -213,6 +213,14 @@ typedef struct mjhOption {
 #define MJH_NXN_TABLE_MAX 100
 
 /* ---- data ---- */
+/* step_desc: optional device buffer of MJH_STEP_DESC_WORDS words (ndesc)
+ * that mjh_step_fused fills with the model / data description each call
+ * (by a one-thread kernel, so the call stays capturable) and its kernels
+ * then read from device memory; a model passed only as a kernel argument
+ * compiles to slower code for the fused kernels (part of it copied to
+ * scratch memory, more registers).  NULL: the kernel-argument path. */
+#define MJH_STEP_DESC_WORDS 2048
+
 #define MJH_DATA_SIZES(X) X(nworld) X(nconmax) X(njmax) X(qM_rows) X(qM_cols) X(qMd_n)
 
 #define MJH_DATA_ARRAYS(X) \
@@ -343,6 +351,7 @@ typedef struct mjhOption {
   X(I, world_con, (nworld, 2)) \
   X(I, world_efc, (nworld, 2)) \
   X(I, world_flags, (nworld)) \
+  X(I, step_desc, (ndesc)) \
   X(I, overflow, (4))
 /* The last five arrays have no counterpart in types.py.  Each world's
  * broadphase pairs, contacts and constraint rows occupy one contiguous block

@@ -85,6 +85,12 @@ def data_node(d, name: str):
   return d, name
 
 
+@functools.lru_cache(maxsize=None)
+def step_desc_words() -> int:
+  """MJH_STEP_DESC_WORDS: the size of Data.step_desc."""
+  return int(re.search(r"#define\s+MJH_STEP_DESC_WORDS\s+(\d+)", HEADER.read_text()).group(1))
+
+
 def model_sizes(m) -> dict:
   msizes, _ = layout()["model"]
   return {s: int(getattr(m, s)) for s in msizes}
@@ -93,7 +99,7 @@ def model_sizes(m) -> dict:
 def data_sizes(m, d) -> dict:
   s = model_sizes(m)
   s.update(nworld=int(d.nworld), nconmax=int(d.nconmax), njmax=int(d.njmax), qM_rows=int(d.qM_rows),
-           qM_cols=int(d.qM_cols), qMd_n=int(d.qMd_n))
+           qM_cols=int(d.qM_cols), qMd_n=int(d.qMd_n), ndesc=int(getattr(d, "ndesc", step_desc_words())))
   s.update(nworld2=2 * s["nworld"], nworld1=s["nworld"] + 1, nworld_ngeom=s["nworld"] * s["ngeom"])
   return s
 

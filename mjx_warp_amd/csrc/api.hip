@@ -514,10 +514,13 @@ int do_integrate(const mjhModel& m, const mjhData& d, hipStream_t s) {
   return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
 }
 
+int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s);
+
 int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
   if (m.opt.integrator != mjhINT_EULER && m.opt.integrator != mjhINT_IMPLICITFAST)
     return fail(MJH_ERR_UNSUPPORTED, "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
   TRY(check_implicit_dense(m));
+  if (d.step_desc && d.nworld > 0) return do_step_fused_desc(m, d, s);
   TRY(reset(d, s, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3));
   if (reg_path(m)) {
     TRY((launch_team<StFusedForward, kRegTeam>(m, d, s, lds_forward(m), "step_fused_forward")));
@@ -625,6 +628,53 @@ int launch_groups(const GroupTable& t, int ldsw, hipStream_t s, const char* name
 }
 
 int max_of(int a, int b) { return a > b ? a : b; }
+
+// The fused step through the group kernels with d as the only group, its
+// description written to d.step_desc first: the kernels then read the
+// model from device memory (the kernel-argument copy of the model compiles
+// to slower code for these kernels: part of it in scratch memory, more
+// registers, one wavefront per SIMD fewer).
+static_assert(sizeof(GroupDesc) <= MJH_STEP_DESC_WORDS * sizeof(int), "step_desc too small");
+static_assert(sizeof(GroupDesc) <= 4000, "the description must fit the kernel arguments");
+__global__ void k_write_desc(GroupDesc g, GroupDesc* out) {
+  if (threadIdx.x == 0) *out = g;
+}
+int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
+  GroupDesc g{};
+  g.m = m;
+  g.d = d;
+  g.first_block = 0;
+  g.tune[0] = tune_for(m);
+  g.tune[1] = reg_path(m) ? tune_tail_reg(m) : g.tune[0];
+  GroupDesc* dev = reinterpret_cast<GroupDesc*>(d.step_desc);
+  const bool reg = reg_path(m);
+  const int l_fwd = lds_forward(m), l_solve = lds_solve_reg(m), l_big = lds_tail_reg(m), l_step = lds_step(m),
+            l_tail = lds_linalg(m);
+  const size_t need = reg ? (size_t)max_of(max_of(2 * l_fwd, 2 * l_solve), max_of(2 * l_big, l_tail))
+                          : (size_t)max_of(l_step, l_tail);
+  if (need * sizeof(float) > 65536)
+    return fail(MJH_ERR_CAPACITY,
+                "step_fused: model too large for the dense LDS kernels (nv = " + std::to_string(m.nv) + ")");
+  hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(64), 0, s, g, dev);
+  TRY(check_launch("step_fused: description"));
+  TRY(reset(d, s, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3));
+  GroupTable t32, t64;
+  t32.dev = t64.dev = dev;
+  t32.ngroups = t64.ngroups = 1;
+  t32.blocks = (d.nworld + 1) / 2;
+  t64.blocks = d.nworld;
+  if (reg) {
+    TRY((launch_groups<kRegTeam, StFusedForward, 0>(t32, l_fwd, s, "step_fused_forward")));
+    TRY(beside(
+        s, [&](hipStream_t q) { return launch_groups<kRegTeam, StBigWorlds<true>, 1>(t32, l_big, q, "step_big_worlds"); },
+        [&](hipStream_t q) {
+          return launch_groups<kRegTeam, StFusedSolveIntegrate, 0>(t32, l_solve, q, "step_fused_solve");
+        }));
+  } else {
+    TRY((launch_groups<kTeam, StFusedStep, 0>(t64, l_step, s, "step_fused")));
+  }
+  return launch_groups<kTeam, StTail<true>, 0>(t64, l_tail, s, "step_tail");
+}
 
 }  // namespace
 

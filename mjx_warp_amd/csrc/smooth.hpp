@@ -445,7 +445,10 @@ MJH_DEV void rne(const mjhModel& m, const mjhData& d, float* lds, const Team<TEA
   const float* qvel = WPTR(d.qvel, w, nv);
   if (t.valid && t.lane < 6) {
     const bool nograv = m.opt.disableflags & mjhDSBL_GRAVITY;
-    const float g = (t.lane < 3 || nograv) ? 0.0f : -m.opt.gravity[t.lane - 3];
+    // constant indices only: a lane-indexed read of the model argument's
+    // array would make the compiler copy the argument to scratch memory
+    const float gl = t.lane == 3 ? m.opt.gravity[0] : (t.lane == 4 ? m.opt.gravity[1] : m.opt.gravity[2]);
+    const float g = (t.lane < 3 || nograv) ? 0.0f : -gl;
     cacc[t.lane] = g;
     caccw[t.lane] = g;
   }

@@ -284,12 +284,14 @@ def _alloc_data(mjm, nworld, nconmax, njmax, device) -> types.Data:
   d.qM_rows, d.qM_cols = (1, mjm.nM) if sparse else (mjm.nv, mjm.nv)
   # dense working copies of qM / qLD: separate arrays for sparse models only
   d.qMd_n = mjm.nv if sparse else 0
+  # device buffer for the fused step's model / data description
+  d.ndesc = _native.step_desc_words()
   m_like = types.Model()
   tables = model_tables(mjm)
   for k in ("nq", "nv", "nu", "na", "nbody", "njnt", "ngeom", "nsite", "nmocap", "nM"):
     setattr(m_like, k, int(getattr(mjm, k)))
   sizes = dict(nworld=d.nworld, nconmax=d.nconmax, njmax=d.njmax, qM_rows=d.qM_rows, qM_cols=d.qM_cols,
-               qMd_n=d.qMd_n, nworld2=2 * d.nworld, nworld1=d.nworld + 1, nworld_ngeom=d.nworld * mjm.ngeom,
+               qMd_n=d.qMd_n, ndesc=d.ndesc, nworld2=2 * d.nworld, nworld1=d.nworld + 1, nworld_ngeom=d.nworld * mjm.ngeom,
                nq=mjm.nq, nv=mjm.nv, nu=mjm.nu, na=mjm.na, nbody=mjm.nbody, njnt=mjm.njnt, ngeom=mjm.ngeom,
                nsite=mjm.nsite, nmocap=mjm.nmocap, nM=mjm.nM, nlsp=int(mjm.opt.ls_iterations))
   _, arrays = _native.layout()["data"]

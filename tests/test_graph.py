@@ -79,3 +79,31 @@ def test_graph_with_big_worlds_matches_eager(fused):
       a = helpers.data_numpy(d_graph, f)
       b = helpers.data_numpy(d_eager, f)
       assert np.array_equal(a, b), (rep, f, np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("fused", [True])
+def test_fused_step_description_path_equals_argument_path(fused):
+  """The fused step reading the model from Data.step_desc (the default) and
+  from its kernel arguments (step_desc left empty) give the same state bit
+  for bit, humanoids standing and lying (with worlds over 64 rows)."""
+  for lying in (False, True):
+    mjm = models.humanoid(iterations=10, ls_iterations=20)
+    if lying:
+      q0 = mjm.key_qpos[0].copy()
+      q0[:3] = [0.0, 0.0, 0.05]
+      q0[3:7] = [np.cos(np.pi / 4), 0.0, np.sin(np.pi / 4), 0.0]
+      q0[7:] = 0.0
+      mjm.key_qpos[0] = q0
+    m = mjw.put_model(mjm)
+    d_desc = _state(mjm, 64, 7, noise=0.2)
+    d_arg = _state(mjm, 64, 7, noise=0.2)
+    d_arg.ndesc = 0
+    d_arg.step_desc = torch.zeros(0, dtype=torch.int32, device="cuda")
+    for _ in range(3):
+      mjw.step_fused(m, d_desc)
+      mjw.step_fused(m, d_arg)
+    torch.cuda.synchronize()
+    for f in ("qpos", "qvel", "qacc", "qacc_warmstart"):
+      a = helpers.data_numpy(d_desc, f)
+      b = helpers.data_numpy(d_arg, f)
+      assert np.array_equal(a, b), (lying, f, np.abs(a - b).max())
