@@ -636,8 +636,13 @@ int max_of(int a, int b) { return a > b ? a : b; }
 // registers, one wavefront per SIMD fewer).
 static_assert(sizeof(GroupDesc) <= MJH_STEP_DESC_WORDS * sizeof(int), "step_desc too small");
 static_assert(sizeof(GroupDesc) <= 4000, "the description must fit the kernel arguments");
-__global__ void k_write_desc(GroupDesc g, GroupDesc* out) {
-  if (threadIdx.x == 0) *out = g;
+// writes the description and clears the step's counters (which as in
+// reset_counters) in one launch
+__global__ void k_write_desc(GroupDesc g, GroupDesc* out, int which) {
+  if (threadIdx.x == 0) {
+    *out = g;
+    reset_counters(g.d, which);
+  }
 }
 int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
   GroupDesc g{};
@@ -655,9 +660,8 @@ int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
   if (need * sizeof(float) > 65536)
     return fail(MJH_ERR_CAPACITY,
                 "step_fused: model too large for the dense LDS kernels (nv = " + std::to_string(m.nv) + ")");
-  hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(64), 0, s, g, dev);
+  hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(64), 0, s, g, dev, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3);
   TRY(check_launch("step_fused: description"));
-  TRY(reset(d, s, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3));
   GroupTable t32, t64;
   t32.dev = t64.dev = dev;
   t32.ngroups = t64.ngroups = 1;
