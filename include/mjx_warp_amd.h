@@ -214,11 +214,12 @@ typedef struct mjhOption {
 
 /* ---- data ---- */
 /* step_desc: optional device buffer of MJH_STEP_DESC_WORDS words (ndesc)
- * that mjh_step_fused fills with the model / data description each call
- * (by a one-thread kernel, so the call stays capturable) and its kernels
- * then read from device memory; a model passed only as a kernel argument
- * compiles to slower code for the fused kernels (part of it copied to
- * scratch memory, more registers).  NULL: the kernel-argument path. */
+ * that mjh_step_fused (and, for nv <= 32, mjh_solve and so mjh_step) fills
+ * with the model / data description each call (by a one-thread kernel, so
+ * the call stays capturable) and whose kernels then read it from device
+ * memory; a model passed only as a kernel argument compiles to slower code
+ * for these kernels (part of it copied to scratch memory, more registers).
+ * NULL: the kernel-argument path, with the same results. */
 #define MJH_STEP_DESC_WORDS 2048
 
 #define MJH_DATA_SIZES(X) X(nworld) X(nconmax) X(njmax) X(qM_rows) X(qM_cols) X(qMd_n)

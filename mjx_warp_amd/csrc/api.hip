@@ -462,8 +462,12 @@ int do_collision(const mjhModel& m, const mjhData& d, hipStream_t s) {
   return launch<StNarrowphase>(m, d, s, 0, "narrowphase");
 }
 
+int solve_reg_desc(const mjhModel& m, const mjhData& d, hipStream_t s);
+
 int do_solve(const mjhModel& m, const mjhData& d, hipStream_t s) {
-  if (reg_path(m)) {
+  if (reg_path(m) && d.step_desc && d.nworld > 0) {
+    TRY(solve_reg_desc(m, d, s));
+  } else if (reg_path(m)) {
     const Tune tt = tune_tail_reg(m);
     TRY(beside(
         s,
@@ -644,6 +648,35 @@ __global__ void k_write_desc(GroupDesc g, GroupDesc* out, int which) {
     reset_counters(g.d, which);
   }
 }
+// The description of d as the only group of a launch.
+GroupDesc single_desc(const mjhModel& m, const mjhData& d) {
+  GroupDesc g{};
+  g.m = m;
+  g.d = d;
+  g.first_block = 0;
+  g.tune[0] = tune_for(m);
+  g.tune[1] = reg_path(m) ? tune_tail_reg(m) : g.tune[0];
+  return g;
+}
+
+// The 32-lane solve launches of the plain step (mjh_solve) through the
+// group kernels, as the fused step does; counters are left alone.
+int solve_reg_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
+  const int l_solve = lds_solve_reg(m), l_big = lds_tail_reg(m);
+  if ((size_t)max_of(2 * l_solve, 2 * l_big) * sizeof(float) > 65536)
+    return fail(MJH_ERR_CAPACITY, "solve: model too large for the dense LDS kernels (nv = " + std::to_string(m.nv) + ")");
+  GroupDesc* dev = reinterpret_cast<GroupDesc*>(d.step_desc);
+  hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(64), 0, s, single_desc(m, d), dev, 0);
+  TRY(check_launch("solve: description"));
+  GroupTable t32;
+  t32.dev = dev;
+  t32.ngroups = 1;
+  t32.blocks = (d.nworld + 1) / 2;
+  return beside(
+      s, [&](hipStream_t q) { return launch_groups<kRegTeam, StBigWorlds<false>, 1>(t32, l_big, q, "solve_big_worlds"); },
+      [&](hipStream_t q) { return launch_groups<kRegTeam, StSolveTeam, 0>(t32, l_solve, q, "solve"); });
+}
+
 int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
   GroupDesc g{};
   g.m = m;

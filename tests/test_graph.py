@@ -81,11 +81,13 @@ def test_graph_with_big_worlds_matches_eager(fused):
       assert np.array_equal(a, b), (rep, f, np.abs(a - b).max())
 
 
-@pytest.mark.parametrize("fused", [True])
+@pytest.mark.parametrize("fused", [True, False])
 def test_fused_step_description_path_equals_argument_path(fused):
-  """The fused step reading the model from Data.step_desc (the default) and
-  from its kernel arguments (step_desc left empty) give the same state bit
-  for bit, humanoids standing and lying (with worlds over 64 rows)."""
+  """The fused step and the plain step's solve read the model from
+  Data.step_desc (the default) or from their kernel arguments (step_desc
+  left empty): the same state bit for bit, humanoids standing and lying
+  (with worlds over 64 rows)."""
+  step = mjw.step_fused if fused else mjw.step
   for lying in (False, True):
     mjm = models.humanoid(iterations=10, ls_iterations=20)
     if lying:
@@ -100,8 +102,8 @@ def test_fused_step_description_path_equals_argument_path(fused):
     d_arg.ndesc = 0
     d_arg.step_desc = torch.zeros(0, dtype=torch.int32, device="cuda")
     for _ in range(3):
-      mjw.step_fused(m, d_desc)
-      mjw.step_fused(m, d_arg)
+      step(m, d_desc)
+      step(m, d_arg)
     torch.cuda.synchronize()
     for f in ("qpos", "qvel", "qacc", "qacc_warmstart"):
       a = helpers.data_numpy(d_desc, f)
