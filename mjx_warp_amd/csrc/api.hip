@@ -18,15 +18,11 @@
 #include <map>
 #include <string>
 #include <vector>
+#include "stages.hpp"
 
-#include "collision.hpp"
-#include "constraint.hpp"
-#include "derivative.hpp"
-#include "forward.hpp"
-#include "smooth.hpp"
-#include "solver.hpp"
-#include "solver_lds.hpp"
-#include "solver_reg.hpp"
+namespace mjh {
+MJH_KERNEL_LISTS(MJH_EXTERN_STAGE, MJH_EXTERN_GROUPS)
+}  // namespace mjh
 
 using namespace mjh;
 
@@ -37,199 +33,6 @@ thread_local std::string g_last_error;
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
-}
-
-constexpr int kTeam = 64;
-
-// launch-time tuning of the LDS solver: row capacity and staged J rows
-struct Tune {
-  int rmax;
-  int rj;
-};
-
-#define STAGE(NAME, CALL)                                                                              \
-  struct NAME {                                                                                        \
-    template <int T>                                                                                   \
-    MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t,          \
-                            const Tune& tune) const {                                                  \
-      (void)l;                                                                                         \
-      (void)tune;                                                                                      \
-      CALL;                                                                                            \
-    }                                                                                                  \
-  };
-
-STAGE(StKinematics, kinematics(m, d, l, t))
-STAGE(StComPos, com_pos(m, d, l, t))
-STAGE(StCrb, crb(m, d, l, t))
-STAGE(StFactorM, factor_m(m, d, l, t))
-STAGE(StRne, rne(m, d, l, t))
-STAGE(StTransmission, transmission(m, d, t))
-STAGE(StComVel, com_vel(m, d, l, t))
-STAGE(StPassive, passive(m, d, t))
-STAGE(StNxn, nxn_broadphase(m, d, l, t))
-STAGE(StSap, sap_broadphase(m, d, l, t))
-STAGE(StBroadphase, if (m.ngeom <= 100) nxn_broadphase(m, d, l, t); else sap_broadphase(m, d, l, t))
-STAGE(StNarrowphase, narrowphase(m, d, t))
-STAGE(StMakeConstraint, make_constraint(m, d, l, t))
-STAGE(StFwdVelocity, fwd_velocity(m, d, l, t))
-STAGE(StFwdActuation, fwd_actuation(m, d, t))
-STAGE(StFwdAcceleration, fwd_acceleration(m, d, l, t))
-STAGE(StEuler, euler(m, d, l, t))
-STAGE(StImplicit, implicit(m, d, l, t))
-// forward.py:676-677 (njmax == 0)
-STAGE(StQaccSmooth, if (t.valid) for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] =
-                        d.qacc_smooth[t.world * m.nv + i])
-// Solver of one world: the register solver on 32-lane teams (nv <= 32),
-// else the LDS solver; false when the world is left for the tail.
-template <int T>
-MJH_DEV bool solve_team(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) {
-  if constexpr (T == kRegTeam) return solve_reg<2>(m, d, l, tune.rj, t);
-  else return solve_lds(m, d, l, tune.rmax, tune.rj, t);
-}
-// Worlds the 32-lane solve launches leave to the big-world launch: more
-// rows than two row sets hold, at most kRegRows.
-MJH_DEV bool big_world(const mjhData& d, int w) {
-  if (d.njmax == 0) return false;
-  const int nr = d.world_efc[2 * w + 1];
-  return nr > 2 * kRegTeam && nr <= kRegRows;
-}
-template <int T>
-MJH_DEV bool left_to_big(const mjhData& d, const Team<T>& t) {
-  if constexpr (T == kRegTeam) return t.valid && big_world(d, t.world);
-  else return false;
-}
-STAGE(StSolveTeam, if (left_to_big(d, t)) return;
-      if (!solve_team(m, d, l, t, tune) && t.lane == 0) d.world_flags[t.world] |= kFlagDeferred)
-// big worlds (32-lane teams, big_world): the register solver with kRegSets
-// row sets; with `step` set also what the fused solve launch does around it
-// (factorisation, smooth acceleration, integrator).  It touches no other
-// world, so it can run beside the solve launch, which skips these worlds.
-template <bool step>
-struct StBigWorlds {
-  template <int T>
-  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
-    static_assert(T == kRegTeam, "big worlds run on 32-lane teams");
-    if (!t.valid || !big_world(d, t.world)) return;
-    if (step) {
-      factor_m(m, d, l, t);
-      sync();
-      fwd_acceleration_solve(m, d, l, t);
-    }
-    solve_reg<kRegSets>(m, d, l, tune.rj, t);
-    sync();
-    if (step) {
-      if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
-      else implicit(m, d, l, t);
-    }
-  }
-};
-
-// tail: flagged worlds get the global-memory solver (and, when `integrate`
-// is set, the integrator), then the flag is cleared
-template <bool integrate>
-struct StTail {
-  template <int T>
-  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune&) const {
-    if (!t.valid || !(d.world_flags[t.world] & kFlagDeferred)) return;
-    solve(m, d, l, t);
-    sync();
-    if (integrate) {
-      if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
-      else implicit(m, d, l, t);
-    }
-    if (t.lane == 0) d.world_flags[t.world] &= ~kFlagDeferred;
-  }
-};
-
-// The whole step of one world (forward.py:682-695 / 664-679).
-struct StFusedStep {
-  template <int T>
-  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
-    kinematics(m, d, l, t);
-    com_pos(m, d, l, t);
-    crb(m, d, l, t);
-    factor_m(m, d, l, t);
-    sync();
-    collision(m, d, l, t);
-    make_constraint(m, d, l, t);
-    sync();
-    transmission(m, d, t);
-    fwd_velocity(m, d, l, t);
-    fwd_actuation(m, d, t);
-    fwd_acceleration(m, d, l, t);
-    if (d.njmax == 0) {
-      if (t.valid)
-        for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] = d.qacc_smooth[t.world * m.nv + i];
-      sync();
-    } else if (!solve_team(m, d, l, t, tune)) {
-      if (t.lane == 0) d.world_flags[t.world] |= kFlagDeferred;
-      return;  // solved and integrated by the tail launch
-    }
-    sync();
-    if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
-    else implicit(m, d, l, t);
-  }
-};
-
-// The fused step in two launches, for 32-lane teams: the forward stages
-// that walk the body tree, collide and build constraint rows (up to
-// qfrc_smooth), then everything that factors or solves with the mass
-// matrix: factor_m, qacc_smooth, the solver and the integrator.  The second
-// group keeps a world's matrix rows in VGPRs; one kernel holding both
-// groups would impose that register budget on the whole step.
-struct StFusedForward {
-  template <int T>
-  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune&) const {
-    kinematics(m, d, l, t);
-    com_pos(m, d, l, t);
-    crb(m, d, l, t);
-    sync();
-    collision(m, d, l, t);
-    make_constraint(m, d, l, t);
-    sync();
-    transmission(m, d, t);
-    fwd_velocity(m, d, l, t);
-    fwd_actuation(m, d, t);
-    fwd_acceleration_force(m, d, t);
-  }
-};
-struct StFusedSolveIntegrate {
-  template <int T>
-  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
-    if (left_to_big(d, t)) return;  // StBigWorlds<true> steps it
-    factor_m(m, d, l, t);
-    sync();
-    fwd_acceleration_solve(m, d, l, t);
-    if (d.njmax == 0) {
-      if (t.valid)
-        for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] = d.qacc_smooth[t.world * m.nv + i];
-      sync();
-    } else if (!solve_team(m, d, l, t, tune)) {
-      if (t.lane == 0) d.world_flags[t.world] |= kFlagDeferred;
-      return;  // solved and integrated by the tail launch
-    }
-    sync();
-    if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
-    else implicit(m, d, l, t);
-  }
-};
-
-template <int TEAM, class S>
-__global__ void __launch_bounds__(64) k_stage(mjhModel m, mjhData d, int lds_per_world, Tune tune) {
-  static_assert(64 % TEAM == 0, "teams tile the wavefront");
-  extern __shared__ float smem[];
-  Team<TEAM> t(d.nworld);
-  S()(m, d, smem + t.slot * lds_per_world, t, tune);
-}
-
-// counter reset without memset nodes (capture-safe):
-// which bit 0: ncollision + ncon, bit 1: nefc
-MJH_DEV void reset_counters(const mjhData& d, int which) {
-  if (which & 1) {
-    d.ncollision[0] = 0;
-    d.ncon[0] = 0;
-  }
-  if (which & 2) d.nefc[0] = 0;
 }
 __global__ void k_reset(mjhData d, int which) {
   if (threadIdx.x == 0) {
@@ -541,39 +344,6 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
     TRY((launch_team<StFusedStep, kTeam>(m, d, s, lds_step(m), "step_fused")));
   }
   return launch_team<StTail<true>, kTeam>(m, d, s, lds_linalg(m), "step_tail");
-}
-
-// ---- heterogeneous batches: several (model, data) groups, each with its
-// own model and worlds, advanced by one launch per stage over all groups ----
-
-// One group as a launch sees it: the group's workgroups are
-// [first_block, first_block + its block count) of the grid.
-struct GroupDesc {
-  mjhModel m;
-  mjhData d;
-  int first_block;
-  Tune tune[2];  // [0] the solve / tail launches, [1] the big-world launch
-};
-
-// The group of workgroup b: the last with first_block <= b (uniform per
-// workgroup; first_block ascends and the first is 0).
-MJH_DEV const GroupDesc& group_of(const GroupDesc* __restrict__ g, int ngroups, int b) {
-  int lo = 0, hi = ngroups - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (g[mid].first_block <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  return g[lo];
-}
-
-template <int TEAM, class S, int TI>
-__global__ void __launch_bounds__(64) k_stage_groups(const GroupDesc* __restrict__ g, int ngroups, int lds_per_world) {
-  extern __shared__ float smem[];
-  const int b = blockIdx.x;
-  const GroupDesc& G = group_of(g, ngroups, b);
-  Team<TEAM> t(G.d.nworld, b - G.first_block);
-  S()(G.m, G.d, smem + t.slot * lds_per_world, t, G.tune[TI]);
 }
 
 // one workgroup per group

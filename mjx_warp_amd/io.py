@@ -488,7 +488,7 @@ def get_data_into(result, mjm, d: types.Data):
   for name in ("dist", "pos", "frame", "includemargin", "friction", "solref", "solreffriction", "solimp", "dim",
                "geom", "efc_address"):
     v = getattr(d.contact, name).cpu().numpy()[:ncon]
-    setattr(c, name, v.reshape(ncon, -1) if v.ndim > 1 else v)
+    setattr(c, name, v.reshape(ncon, int(np.prod(v.shape[1:]))) if v.ndim > 1 else v)
   result.contact = c
   for name in ("D", "pos", "aref", "force", "margin"):
     setattr(result, "efc_" + name, getattr(d.efc, name).cpu().numpy()[:nefc].astype(np.float64))

@@ -118,3 +118,52 @@ def oracle_stepper(mjm, m_cpu, real=np.float64, nconmax_w=64, njmax_w=277):
             hd["qacc_warmstart"].astype(np.float64))
 
   return step
+
+
+def mjdata_from_oracle(mjm, hd, w=0):
+  """A host MjData (MuJoCo's field names and host layouts) holding world w
+  of an oracle HostData after a forward pass: the state, the derived
+  fields, qM in MuJoCo's sparse layout, the world's contacts and
+  constraint rows -- what mj_forward leaves in an mjData, the input of the
+  reference's put_data (test_util.py:31-44)."""
+  from mjx_warp_amd import mjcf
+
+  mjd = mjcf.MjData(mjm)
+  nb, nv = mjm.nbody, mjm.nv
+  for name in ("qpos", "qvel", "qacc", "qacc_warmstart", "qfrc_applied", "ctrl", "act", "act_dot", "xpos",
+               "xquat", "xipos", "xanchor", "xaxis", "subtree_com", "geom_xpos", "site_xpos", "cinert", "cdof",
+               "crb", "cvel", "cdof_dot", "actuator_length", "actuator_velocity", "actuator_force", "qfrc_bias",
+               "qfrc_spring", "qfrc_damper", "qfrc_passive", "qfrc_actuator", "qfrc_smooth", "qfrc_constraint",
+               "qacc_smooth", "xfrc_applied", "mocap_pos", "mocap_quat", "qLDiagInv"):
+    setattr(mjd, name, np.asarray(hd[name][w], np.float64).copy())
+  for name in ("xmat", "ximat", "geom_xmat", "site_xmat"):
+    setattr(mjd, name, np.asarray(hd[name][w], np.float64).reshape(-1, 9).copy())
+  mjd.actuator_moment = np.asarray(hd["actuator_moment"][w], np.float64).copy()
+  # dense M -> MuJoCo's sparse qM (row i: i, then its ancestors' columns)
+  M = np.asarray(hd["qMd"][w] if hd.sizes["qMd_n"] else hd["qM"][w], np.float64)
+  sq = []
+  for i in range(nv):
+    j = i
+    while j >= 0:
+      sq.append(M[i, j])
+      j = mjm.dof_parentid[j]
+  mjd.qM = np.array(sq)
+  c0, nc = (int(x) for x in hd["world_con"].reshape(-1, 2)[w])
+
+  class _Contact:
+    pass
+
+  c = _Contact()
+  for name, kind in (("dist", 1), ("pos", 3), ("frame", 9), ("includemargin", 1), ("friction", 5), ("solref", 2),
+                     ("solreffriction", 2), ("solimp", 5), ("dim", 1), ("geom", 2), ("efc_address", 1)):
+    v = np.asarray(hd["contact_" + name])
+    v = v.reshape(v.shape[0], -1)[c0:c0 + nc]
+    setattr(c, name, v if kind > 1 else v[:, 0])
+  mjd.contact = c
+  mjd.ncon = nc
+  r0, nr = (int(x) for x in hd["world_efc"].reshape(-1, 2)[w])
+  mjd.nefc = nr
+  mjd.efc_J = np.asarray(hd["efc_J"][r0:r0 + nr], np.float64).copy()
+  for name in ("D", "pos", "aref", "force", "margin"):
+    setattr(mjd, "efc_" + name, np.asarray(hd["efc_" + name][r0:r0 + nr], np.float64).copy())
+  return mjd

@@ -1,0 +1,190 @@
+"""Parity over the reference benchmark's horizon of 1000 steps
+(testspeed.py:42, test_util.py:99-118).
+
+Two workloads: the 3-link chain (64 worlds, CG 100, Euler dt 0.002, qpos ~
+U(-0.5, 0.5), qvel ~ N(0, 0.1)) and the humanoid of the speed benchmark (64
+worlds from the squat keyframe, gears 100, Newton 10 / 20, implicitfast dt
+0.005, ctrl ~ U(-1, 1) drawn anew every step).  Each runs 1000 steps on the
+GPU, plain and fused.
+
+* Re-synced parity: every 50 steps the oracle takes one step from the GPU's
+  own state (qpos, qvel, qacc_warmstart, ctrl) and its qpos, qvel, qacc and
+  per-world efc_force are compared with the GPU's next state -- as the
+  single-step tests do, with the float32 oracle's rounding envelope
+  (float64 and perturbed float32 runs) for worlds whose truncated solve is
+  rounding-sensitive.
+* Free-running envelope at step 1000: the float32 oracle runs all 1000 steps
+  from the same initial state and controls, beside the float64 oracle and
+  two float32 runs whose hinge coordinates start 1e-6 apart.  Contact-rich
+  trajectories are chaotic, so after 1000 steps these runs differ from each
+  other; the GPU's distance from the float32 run must be of the size of the
+  oracle's own spread (at most 3x, plus a small absolute slack).
+
+The maximum errors against the float32 oracle (parity) and against the
+float64 oracle (accuracy) are printed per field.
+"""
+
+import json
+
+import numpy as np
+import pytest
+import torch
+
+import mjx_warp_amd as mjw
+from mjx_warp_amd import models
+from oracle import oracle
+
+import helpers
+from test_gpu_parity import _close_or_enveloped
+
+pytestmark = pytest.mark.gpu
+
+NSTEP = 1000
+EVERY = 50
+
+
+def _workload(name):
+  if name == "chain":
+    mjm = models.chain(3)
+    nworld = 64
+    q, v = models.chain_initial_state(nworld, seed=0)
+    return mjm, nworld, q, v, None, 8, 35
+  mjm = models.humanoid(iterations=10, ls_iterations=20)
+  nworld = 64
+  q = np.tile(mjm.key_qpos[0], (nworld, 1))
+  v = np.zeros((nworld, mjm.nv))
+  ctrl = np.random.default_rng(1).uniform(-1, 1, (NSTEP, nworld, mjm.nu)).astype(np.float32)
+  return mjm, nworld, q, v, ctrl, 64, 277
+
+
+class OracleRun:
+  """An oracle Data of the workload's size in float32 or float64."""
+
+  def __init__(self, mjm, nworld, nconmax_w, njmax_w, real):
+    m = mjw.put_model(mjm, device="cpu")
+    d = mjw.make_data(mjm, nworld=nworld, nconmax=nconmax_w * nworld, njmax=njmax_w * nworld, device="cpu")
+    self.hm = helpers.host_model(m, real) if real == np.float32 else helpers.host_model_exact(mjm, m, real)
+    self.hd = helpers.host_data(m, d, real)
+    self.nu = mjm.nu
+
+  def set(self, qpos, qvel, warm=None):
+    self.hd.arrays["qpos"][:] = qpos
+    self.hd.arrays["qvel"][:] = qvel
+    self.hd.arrays["qacc_warmstart"][:] = 0.0 if warm is None else warm
+
+  def step(self, ctrl=None):
+    if self.nu and ctrl is not None:
+      self.hd.arrays["ctrl"][:] = ctrl
+    oracle.run("step", self.hm, self.hd, nthreads=8)
+
+  def fields(self):
+    h = self.hd
+    return dict(qpos=h["qpos"].astype(np.float64).copy(), qvel=h["qvel"].astype(np.float64).copy(),
+                qacc=h["qacc"].astype(np.float64).copy(), world_efc=h["world_efc"].copy(),
+                efc_force=h["efc_force"].astype(np.float64).copy())
+
+
+def _efc_by_world(f, nworld):
+  blocks = helpers.world_blocks(f["world_efc"], nworld)
+  return [f["efc_force"][b:b + n] for b, n in blocks]
+
+
+def _gpu_fields(d):
+  return dict(qpos=helpers.data_numpy(d, "qpos").astype(np.float64), qvel=helpers.data_numpy(d, "qvel").astype(np.float64),
+              qacc=helpers.data_numpy(d, "qacc").astype(np.float64), world_efc=helpers.data_numpy(d, "world_efc"),
+              efc_force=helpers.data_numpy(d, "efc_force").astype(np.float64),
+              warm=helpers.data_numpy(d, "qacc_warmstart").astype(np.float64))
+
+
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("name", ["chain", "humanoid"])
+def test_thousand_steps(name, fused):
+  mjm, nworld, q0, v0, ctrl, nconmax_w, njmax_w = _workload(name)
+  m = mjw.put_model(mjm)
+  d = mjw.make_data(mjm, nworld=nworld, nconmax=nconmax_w * nworld, njmax=njmax_w * nworld)
+  helpers.set_field(d, "qpos", q0)
+  helpers.set_field(d, "qvel", v0)
+  ctrl_dev = torch.tensor(ctrl, device="cuda") if ctrl is not None else None
+  step = mjw.step_fused if fused else mjw.step
+
+  o32 = OracleRun(mjm, nworld, nconmax_w, njmax_w, np.float32)
+  o64 = OracleRun(mjm, nworld, nconmax_w, njmax_w, np.float64)
+  hinge = [mjm.jnt_qposadr[j] for j in range(mjm.njnt) if mjm.jnt_type[j] in (2, 3)]
+  worst32 = {k: 0.0 for k in ("qpos", "qvel", "qacc", "efc_force")}
+  worst64 = dict(worst32)
+  rows_differ = 0
+  for t in range(NSTEP):
+    snap = t % EVERY == 0
+    if snap:
+      torch.cuda.synchronize()
+      before = _gpu_fields(d)
+    if ctrl_dev is not None:
+      d.ctrl.copy_(ctrl_dev[t])
+    step(m, d)
+    if not snap:
+      continue
+    torch.cuda.synchronize()
+    after = _gpu_fields(d)
+    u = ctrl[t] if ctrl is not None else None
+    ref = {}
+    for key, orun in (("f32", o32), ("f64", o64)):
+      orun.set(before["qpos"], before["qvel"], before["warm"])
+      orun.step(u)
+      ref[key] = orun.fields()
+    # the float32 oracle's rounding envelope at this state
+    rng = np.random.default_rng(t)
+    env = {k: [ref["f64"][k]] for k in ("qpos", "qvel", "qacc")}
+    for _ in range(4):
+      qp = before["qpos"].copy()
+      qp[:, hinge] += 1e-6 * rng.normal(size=(nworld, len(hinge)))
+      o32.set(qp, before["qvel"] * (1 + 1e-6 * rng.normal(size=before["qvel"].shape)), before["warm"])
+      o32.step(u)
+      f = o32.fields()
+      for k in env:
+        env[k].append(f[k])
+    for k, tol in (("qpos", 1e-4), ("qvel", 2e-3), ("qacc", 2e-3)):
+      _close_or_enveloped(f"{k} at step {t}", after[k], ref["f32"][k], np.stack(env[k]), tol)
+      worst32[k] = max(worst32[k], float(np.abs(after[k] - ref["f32"][k]).max()))
+      worst64[k] = max(worst64[k], float(np.abs(after[k] - ref["f64"][k]).max()))
+    g = _efc_by_world(after, nworld)
+    for key, worst in (("f32", worst32), ("f64", worst64)):
+      r = _efc_by_world(ref[key], nworld)
+      for a, b in zip(g, r):
+        if len(a) != len(b):
+          rows_differ += key == "f32"
+          continue
+        if len(a):
+          worst["efc_force"] = max(worst["efc_force"], float(np.abs(a - b).max()))
+  torch.cuda.synchronize()
+  final = _gpu_fields(d)
+  assert np.isfinite(final["qpos"]).all() and np.isfinite(final["qvel"]).all()
+  # a re-synced state differs in its row count only where a contact sits at
+  # the activation margin within rounding
+  assert rows_differ <= nworld * (NSTEP // EVERY) // 50, rows_differ
+
+  # free-running: the oracle from the initial state, 1000 steps
+  runs = []
+  rng = np.random.default_rng(99)
+  for i, real in enumerate((np.float32, np.float64, np.float32, np.float32)):
+    orun = OracleRun(mjm, nworld, nconmax_w, njmax_w, real)
+    q = q0.copy()
+    if i >= 2:
+      q[:, hinge] += 1e-6 * rng.normal(size=(nworld, len(hinge)))
+    orun.set(q, v0)
+    for t in range(NSTEP):
+      orun.step(ctrl[t] if ctrl is not None else None)
+    runs.append(orun.fields())
+  free = {}
+  for k in ("qpos", "qvel"):
+    ref32 = runs[0][k]
+    err = np.abs(final[k] - ref32).max(axis=1)
+    spread = np.max([np.abs(r[k] - ref32).max(axis=1) for r in runs[1:]], axis=0)
+    slack = 1e-3 * (1 + np.abs(ref32).max())
+    inside = err <= 3 * spread + slack
+    assert np.mean(inside) >= 0.9, (k, err[~inside], spread[~inside])
+    free[k] = dict(gpu_vs_f32_max=float(err.max()), gpu_vs_f32_median=float(np.median(err)),
+                   gpu_vs_f64_max=float(np.abs(final[k] - runs[1][k]).max()),
+                   oracle_spread_median=float(np.median(spread)), worlds_inside=float(np.mean(inside)))
+  print("LONG_PARITY " + json.dumps(dict(workload=name, fused=fused, steps=NSTEP, every=EVERY,
+                                         resync_max_err_vs_f32=worst32, resync_max_err_vs_f64=worst64,
+                                         rows_differ=rows_differ, free_running_at_1000=free)), flush=True)

@@ -641,3 +641,108 @@ def test_contact_rows_are_contact_point_velocities():
   for k, rows in enumerate(((0, 1), (2, 3))):
     jt = (J[rows[0]] - J[rows[1]]) / (2 * mu)
     np.testing.assert_allclose(jt @ v, frame[1 + k] @ vp, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# mj_setConst of an articulated body: a planar two-link arm
+def test_set_const_planar_two_link_arm():
+  """Two hinges about y; link i a sphere of mass m_i, radius r_i at c_i
+  along x from its hinge; the second hinge l1 along x from the first.  At
+  qpos0 (straight arm) the textbook double-pendulum inertia is
+    M11 = I1 + m1 c1^2 + I2 + m2 (l1 + c2)^2,
+    M12 = I2 + m2 c2 (l1 + c2),  M22 = I2 + m2 c2^2,   I_i = 2/5 m_i r_i^2;
+  dof_invweight0 = diag(M^-1), meaninertia = tr(M) / 2, and body
+  invweights are the traces of J M^-1 J' over 3 at each body's COM: the
+  COM of link 2 moves along z by (l1 + c2, c2) per unit hinge rate and
+  spins about y by (1, 1); link 1 by (c1, 0) and (1, 0)."""
+  m1, r1, c1, l1 = 1.3, 0.07, 0.25, 0.6
+  m2, r2, c2 = 0.8, 0.05, 0.3
+  mjm = mjcf.from_xml_string(f"""<mujoco><worldbody>
+    <body pos="0 0 2"><joint type="hinge" axis="0 1 0"/>
+      <geom type="sphere" pos="{c1} 0 0" size="{r1}" mass="{m1}"/>
+      <body pos="{l1} 0 0"><joint type="hinge" axis="0 1 0"/>
+        <geom type="sphere" pos="{c2} 0 0" size="{r2}" mass="{m2}"/></body></body>
+    </worldbody></mujoco>""")
+  I1, I2 = 0.4 * m1 * r1 * r1, 0.4 * m2 * r2 * r2
+  M = np.array([[I1 + m1 * c1 ** 2 + I2 + m2 * (l1 + c2) ** 2, I2 + m2 * c2 * (l1 + c2)],
+                [I2 + m2 * c2 * (l1 + c2), I2 + m2 * c2 ** 2]])
+  Minv = np.linalg.inv(M)
+  np.testing.assert_allclose(mjm.stat.meaninertia, np.trace(M) / 2, rtol=1e-9)
+  np.testing.assert_allclose(mjm.dof_invweight0, np.diag(Minv), rtol=1e-9)
+  jt2, jr2 = np.array([l1 + c2, c2]), np.array([1.0, 1.0])
+  np.testing.assert_allclose(mjm.body_invweight0[2], [jt2 @ Minv @ jt2 / 3, jr2 @ Minv @ jr2 / 3], rtol=1e-9)
+  np.testing.assert_allclose(mjm.body_invweight0[1], [c1 * c1 * Minv[0, 0] / 3, Minv[0, 0] / 3], rtol=1e-9)
+  # and the oracle's mass matrix at qpos0 is the same matrix
+  mjm, m, d, hm, hd = helpers.load(mjm)
+  for s in ("kinematics", "com_pos", "crb"):
+    oracle.run(s, hm, hd)
+  np.testing.assert_allclose(hd["qM"][0], M, rtol=1e-9, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# forward kinematics in closed form
+def test_hinge_chain_kinematics_closed_form():
+  """3-link chain hung from (0, 0, 1.5), hinges about y, 0.5 m links along
+  -z: with cumulative angles T_k = q_1 + ... + q_k, body k turns by R_y(T_k)
+  (quaternion (cos T_k/2, 0, sin T_k/2, 0)), the next hinge sits 0.5 m along
+  R_y(T_k)(0, 0, -1) = (-sin T_k, 0, -cos T_k), and each link's COM (and
+  capsule centre) halfway down it."""
+  mjm = models.chain(3)
+  rng = np.random.default_rng(11)
+  nworld = 8
+  mjm, m, d, hm, hd = helpers.load(mjm, nworld=nworld)
+  q = rng.uniform(-1.5, 1.5, (nworld, 3))
+  hd.arrays["qpos"][:] = q
+  oracle.run("kinematics", hm, hd)
+  for w in range(nworld):
+    T = np.cumsum(q[w])
+    p = np.array([0.0, 0.0, 1.5])
+    for k in range(3):
+      b = k + 1
+      down = np.array([-np.sin(T[k]), 0.0, -np.cos(T[k])])
+      R = np.array([[np.cos(T[k]), 0, np.sin(T[k])], [0, 1, 0], [-np.sin(T[k]), 0, np.cos(T[k])]])
+      np.testing.assert_allclose(hd["xpos"][w, b], p, atol=1e-12)
+      quat = np.array([np.cos(T[k] / 2), 0.0, np.sin(T[k] / 2), 0.0])
+      np.testing.assert_allclose(hd["xquat"][w, b] * np.sign(hd["xquat"][w, b, 0]), quat * np.sign(quat[0]),
+                                 atol=1e-12)
+      np.testing.assert_allclose(hd["xmat"][w, b].reshape(3, 3), R, atol=1e-12)
+      np.testing.assert_allclose(hd["xanchor"][w, k], p, atol=1e-12)
+      np.testing.assert_allclose(hd["xaxis"][w, k], [0, 1, 0], atol=1e-12)
+      np.testing.assert_allclose(hd["xipos"][w, b], p + 0.25 * down, atol=1e-12)
+      np.testing.assert_allclose(hd["geom_xpos"][w, b], p + 0.25 * down, atol=1e-12)
+      p = p + 0.5 * down
+
+
+def test_ball_and_free_joint_kinematics_closed_form():
+  """A free body carrying a ball-jointed pendulum: the free body's pose is
+  qpos[0:7]; the child (at offset o in the parent, its COM at c below the
+  ball) has orientation q_free * q_ball and COM x + R(q_free) (o + R(q_ball) c),
+  rotations from scipy (an independent quaternion convention check)."""
+  from scipy.spatial.transform import Rotation
+
+  o, c = np.array([0.1, -0.2, 0.05]), np.array([0.0, 0.0, -0.4])
+  mjm = mjcf.from_xml_string(f"""<mujoco><worldbody>
+    <body pos="0 0 1"><freejoint/><geom type="sphere" size="0.1"/>
+      <body pos="{o[0]} {o[1]} {o[2]}"><joint type="ball"/>
+        <geom type="sphere" pos="{c[0]} {c[1]} {c[2]}" size="0.05"/></body></body>
+    </worldbody></mujoco>""")
+  rng = np.random.default_rng(12)
+  nworld = 6
+  mjm, m, d, hm, hd = helpers.load(mjm, nworld=nworld)
+  qf = rng.normal(size=(nworld, 4))
+  qf /= np.linalg.norm(qf, axis=1, keepdims=True)
+  qb = rng.normal(size=(nworld, 4))
+  qb /= np.linalg.norm(qb, axis=1, keepdims=True)
+  x = rng.normal(size=(nworld, 3))
+  hd.arrays["qpos"][:] = np.concatenate([x, qf, qb], axis=1)
+  oracle.run("kinematics", hm, hd)
+  for w in range(nworld):
+    Rf = Rotation.from_quat(np.roll(qf[w], -1))  # scipy takes (x, y, z, w)
+    Rb = Rotation.from_quat(np.roll(qb[w], -1))
+    np.testing.assert_allclose(hd["xpos"][w, 1], x[w], atol=1e-12)
+    np.testing.assert_allclose(hd["xmat"][w, 1].reshape(3, 3), Rf.as_matrix(), atol=1e-12)
+    child = Rf * Rb
+    np.testing.assert_allclose(hd["xmat"][w, 2].reshape(3, 3), child.as_matrix(), atol=1e-12)
+    np.testing.assert_allclose(hd["xpos"][w, 2], x[w] + Rf.apply(o), atol=1e-12)
+    np.testing.assert_allclose(hd["xipos"][w, 2], x[w] + Rf.apply(o + Rb.apply(c)), atol=1e-12)
+    np.testing.assert_allclose(hd["xanchor"][w, 1], x[w] + Rf.apply(o), atol=1e-12)
