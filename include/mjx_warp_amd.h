@@ -485,6 +485,20 @@ int mjh_multi_step(const mjhMulti* plan, mjhStream stream);
 /* Frees the plan (after its last step has finished). */
 int mjh_multi_destroy(mjhMulti* plan);
 
+/* No counterpart in the reference.  The same step as mjh_multi_create +
+ * mjh_multi_step without a plan and without any allocation, for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the group table lives in
+ * a fixed device array of the library (one per GPU, held until the process
+ * ends) that the call fills with one-thread kernels on `stream` before the
+ * step's launches; the big-world launch runs in order on `stream` (no
+ * second stream).  At most MJH_MULTI_STATIC_MAX_GROUPS groups (larger
+ * counts return MJH_ERR_CAPACITY).  Every call on a GPU rewrites that GPU's
+ * array: calls that overlap on one GPU (two streams, two host threads) race
+ * -- order them on one stream or with events. */
+#define MJH_MULTI_STATIC_MAX_GROUPS 64
+int mjh_multi_step_static(int ngroups, const mjhModel* models, const mjhData* datas, mjhStream stream);
+
 /* ---- transition derivatives and the adjoint ----
  * The reference has no gradients (its kernels overwrite arrays in place;
  * collision_convex.py:22 disables backward code generation); these follow

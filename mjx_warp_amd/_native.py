@@ -200,6 +200,7 @@ def lib():
   L.mjh_rollout.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P]
   L.mjh_multi_create.argtypes = [ctypes.c_int, P, P, P]
   L.mjh_multi_step.argtypes = [P, P]
+  L.mjh_multi_step_static.argtypes = [ctypes.c_int, P, P, P]
   L.mjh_multi_destroy.argtypes = [P]
   L.mjh_transition_fd.argtypes = [P, P, P, ctypes.c_float, ctypes.c_int, P, P, P]
   L.mjh_adjoint_step.argtypes = [P, P, P, P, P, P, P, P, P, P, P]

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "stages.hpp"
@@ -215,10 +216,10 @@ int check_hip(hipError_t e, const char* what) {
 
 // Runs side(stream) beside main(s): side on the side stream, forked after
 // the work already queued on s and joined before what s runs next; in order
-// on s when there is no side stream.
+// on s when there is no side stream or use_side is false.
 template <class Side, class Main>
-int beside(hipStream_t s, Side side, Main main) {
-  SideStream* ss = side_stream();
+int beside(hipStream_t s, Side side, Main main, bool use_side = true) {
+  SideStream* ss = use_side ? side_stream() : nullptr;
   if (!ss) {
     TRY(main(s));
     return side(s);
@@ -483,6 +484,95 @@ int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
   return launch_groups<kTeam, StTail<true>, 0>(t64, l_tail, s, "step_tail");
 }
 
+// Checks group i (as mjh_check and the fused step do), fills its table
+// entry (model, data, tuning; first_block is the caller's) and folds its
+// per-world LDS strides into plan.  reg: the group runs on 32-lane teams.
+int plan_group(int i, const mjhModel& m, const mjhData& d, mjhMulti* plan, GroupDesc* gd, bool* reg) {
+  if (mjh_check(&m, &d) != MJH_OK) return fail(MJH_ERR_INVALID, "multi: group " + std::to_string(i) + ": " + g_last_error);
+  if (m.opt.integrator != mjhINT_EULER && m.opt.integrator != mjhINT_IMPLICITFAST)
+    return fail(MJH_ERR_UNSUPPORTED, "multi: group " + std::to_string(i) + ": integrator " +
+                                         std::to_string(m.opt.integrator) + " not implemented.");
+  TRY(check_implicit_dense(m));
+  gd->m = m;
+  gd->d = d;
+  gd->first_block = 0;
+  gd->tune[0] = tune_for(m);
+  *reg = reg_path(m);
+  if (*reg) {
+    gd->tune[1] = tune_tail_reg(m);
+    plan->lds_fwd = max_of(plan->lds_fwd, lds_forward(m));
+    plan->lds_solve = max_of(plan->lds_solve, lds_solve_reg(m));
+    plan->lds_big = max_of(plan->lds_big, lds_tail_reg(m));
+    plan->lds_tail32 = max_of(plan->lds_tail32, lds_linalg(m));
+  } else {
+    gd->tune[1] = gd->tune[0];
+    plan->lds_step = max_of(plan->lds_step, lds_step(m));
+    plan->lds_tail64 = max_of(plan->lds_tail64, lds_linalg(m));
+  }
+  return MJH_OK;
+}
+
+// each launch's LDS: two worlds per workgroup on 32-lane teams, one on 64
+int plan_capacity(const mjhMulti& plan) {
+  const int need = max_of(max_of(2 * plan.lds_fwd, 2 * plan.lds_solve), max_of(2 * plan.lds_big, plan.lds_tail32));
+  if ((size_t)max_of(need, max_of(plan.lds_step, plan.lds_tail64)) * sizeof(float) > 65536)
+    return fail(MJH_ERR_CAPACITY, "multi: a group's model is too large for the dense LDS kernels");
+  return MJH_OK;
+}
+
+// The launches of one fused step of every group of a plan; the big-world
+// launch beside the solve launch when use_side, else in order on s.
+int multi_launches(const mjhMulti& p, hipStream_t s, bool use_side) {
+  for (const GroupTable* t : {&p.t32, &p.t64}) {
+    if (t->ngroups == 0) continue;
+    hipLaunchKernelGGL(k_reset_groups, dim3(t->ngroups), dim3(64), 0, s, t->dev);
+    TRY(check_launch("multi: reset"));
+  }
+  if (p.t32.ngroups) {
+    TRY((launch_groups<kRegTeam, StFusedForward, 0>(p.t32, p.lds_fwd, s, "multi: forward")));
+    TRY(beside(
+        s,
+        [&](hipStream_t q) {
+          return launch_groups<kRegTeam, StBigWorlds<true>, 1>(p.t32, p.lds_big, q, "multi: big worlds");
+        },
+        [&](hipStream_t q) {
+          return launch_groups<kRegTeam, StFusedSolveIntegrate, 0>(p.t32, p.lds_solve, q, "multi: solve");
+        },
+        use_side));
+    TRY((launch_groups<kTeam, StTail<true>, 0>(p.t32_64, p.lds_tail32, s, "multi: tail")));
+  }
+  if (p.t64.ngroups) {
+    TRY((launch_groups<kTeam, StFusedStep, 0>(p.t64, p.lds_step, s, "multi: step")));
+    TRY((launch_groups<kTeam, StTail<true>, 0>(p.t64, p.lds_tail64, s, "multi: tail")));
+  }
+  return MJH_OK;
+}
+
+// The group table of mjh_multi_step_static: a fixed device array (each GPU
+// holds its own copy at its own address), so the call allocates nothing.
+// The three tables of a plan sit back to back: 2 n32 + n64 <= 2 ngroups
+// entries.
+static __device__ __align__(256) GroupDesc scratch[2 * MJH_MULTI_STATIC_MAX_GROUPS];
+constexpr int kMaxDevices = 64;
+GroupDesc* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// The device address of this GPU's table, looked up once per device with
+// that device current; host code uses it, never the array's name.
+int static_table(GroupDesc** out) {
+  int dev = 0;
+  TRY(check_hip(hipGetDevice(&dev), "multi_static: device"));
+  if (dev < 0 || dev >= kMaxDevices) return fail(MJH_ERR_CAPACITY, "multi_static: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    TRY(check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "multi_static: table address"));
+    g_scratch_ptr[dev] = static_cast<GroupDesc*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return MJH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -603,35 +693,12 @@ int mjh_multi_create(int ngroups, const mjhModel* models, const mjhData* datas, 
   std::vector<GroupDesc> g32, g64;
   mjhMulti plan;
   for (int i = 0; i < ngroups; i++) {
-    const mjhModel& m = models[i];
-    const mjhData& d = datas[i];
-    const std::string at = "multi: group " + std::to_string(i) + ": ";
-    if (mjh_check(&m, &d) != MJH_OK) return fail(MJH_ERR_INVALID, at + g_last_error);
-    if (m.opt.integrator != mjhINT_EULER && m.opt.integrator != mjhINT_IMPLICITFAST)
-      return fail(MJH_ERR_UNSUPPORTED, at + "integrator " + std::to_string(m.opt.integrator) + " not implemented.");
-    TRY(check_implicit_dense(m));
     GroupDesc gd{};
-    gd.m = m;
-    gd.d = d;
-    gd.tune[0] = tune_for(m);
-    if (reg_path(m)) {
-      gd.tune[1] = tune_tail_reg(m);
-      g32.push_back(gd);
-      plan.lds_fwd = max_of(plan.lds_fwd, lds_forward(m));
-      plan.lds_solve = max_of(plan.lds_solve, lds_solve_reg(m));
-      plan.lds_big = max_of(plan.lds_big, lds_tail_reg(m));
-      plan.lds_tail32 = max_of(plan.lds_tail32, lds_linalg(m));
-    } else {
-      gd.tune[1] = gd.tune[0];
-      g64.push_back(gd);
-      plan.lds_step = max_of(plan.lds_step, lds_step(m));
-      plan.lds_tail64 = max_of(plan.lds_tail64, lds_linalg(m));
-    }
+    bool reg = false;
+    TRY(plan_group(i, models[i], datas[i], &plan, &gd, &reg));
+    (reg ? g32 : g64).push_back(gd);
   }
-  // each launch's LDS: two worlds per workgroup on 32-lane teams, one on 64
-  const int need = max_of(max_of(2 * plan.lds_fwd, 2 * plan.lds_solve), max_of(2 * plan.lds_big, plan.lds_tail32));
-  if ((size_t)max_of(need, max_of(plan.lds_step, plan.lds_tail64)) * sizeof(float) > 65536)
-    return fail(MJH_ERR_CAPACITY, "multi: a group's model is too large for the dense LDS kernels");
+  TRY(plan_capacity(plan));
   mjhMulti* p = new mjhMulti(plan);
   int rc = build_table(g32, 2, &p->t32);
   if (rc == MJH_OK) rc = build_table(g32, 1, &p->t32_64);
@@ -646,29 +713,60 @@ int mjh_multi_create(int ngroups, const mjhModel* models, const mjhData* datas, 
 }
 int mjh_multi_step(const mjhMulti* p, mjhStream stream) {
   if (!p) return fail(MJH_ERR_INVALID, "multi: null plan");
+  return multi_launches(*p, (hipStream_t)stream, true);
+}
+int mjh_multi_step_static(int ngroups, const mjhModel* models, const mjhData* datas, mjhStream stream) {
+  if (ngroups <= 0 || !models || !datas) return fail(MJH_ERR_INVALID, "multi_static: no groups");
+  if (ngroups > MJH_MULTI_STATIC_MAX_GROUPS)
+    return fail(MJH_ERR_CAPACITY, "multi_static: " + std::to_string(ngroups) + " groups; at most " +
+                                      std::to_string(MJH_MULTI_STATIC_MAX_GROUPS) + " fit the library's table");
+  mjhMulti plan;
+  int n32 = 0, n64 = 0;
+  for (int i = 0; i < ngroups; i++) {
+    GroupDesc gd{};
+    bool reg = false;
+    TRY(plan_group(i, models[i], datas[i], &plan, &gd, &reg));
+    (reg ? n32 : n64)++;
+  }
+  TRY(plan_capacity(plan));
+  GroupDesc* table = nullptr;
+  TRY(static_table(&table));
   hipStream_t s = (hipStream_t)stream;
-  for (const GroupTable* t : {&p->t32, &p->t64}) {
-    if (t->ngroups == 0) continue;
-    hipLaunchKernelGGL(k_reset_groups, dim3(t->ngroups), dim3(64), 0, s, t->dev);
-    TRY(check_launch("multi: reset"));
+  // the three tables of a plan, back to back: 32-lane launches (two worlds
+  // per workgroup), the 64-lane tail over the same groups, 64-lane groups
+  plan.t32.dev = table;
+  plan.t32_64.dev = table + n32;
+  plan.t64.dev = table + 2 * n32;
+  plan.t32.ngroups = plan.t32_64.ngroups = n32;
+  plan.t64.ngroups = n64;
+  long long b32 = 0, b32_64 = 0, b64 = 0;
+  int i32 = 0, i64 = 0;
+  for (int i = 0; i < ngroups; i++) {
+    GroupDesc gd{};
+    bool reg = false;
+    TRY(plan_group(i, models[i], datas[i], &plan, &gd, &reg));
+    const int nw = datas[i].nworld;
+    if (reg) {
+      gd.first_block = (int)b32;
+      hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(64), 0, s, gd, plan.t32.dev + i32, 0);
+      gd.first_block = (int)b32_64;
+      hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(64), 0, s, gd, plan.t32_64.dev + i32, 0);
+      b32 += (nw + 1) / 2;
+      b32_64 += nw;
+      i32++;
+    } else {
+      gd.first_block = (int)b64;
+      hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(64), 0, s, gd, plan.t64.dev + i64, 0);
+      b64 += nw;
+      i64++;
+    }
+    TRY(check_launch("multi_static: table"));
+    if (b32_64 > 0x7fffffffLL || b64 > 0x7fffffffLL) return fail(MJH_ERR_CAPACITY, "multi_static: too many worlds");
   }
-  if (p->t32.ngroups) {
-    TRY((launch_groups<kRegTeam, StFusedForward, 0>(p->t32, p->lds_fwd, s, "multi: forward")));
-    TRY(beside(
-        s,
-        [&](hipStream_t q) {
-          return launch_groups<kRegTeam, StBigWorlds<true>, 1>(p->t32, p->lds_big, q, "multi: big worlds");
-        },
-        [&](hipStream_t q) {
-          return launch_groups<kRegTeam, StFusedSolveIntegrate, 0>(p->t32, p->lds_solve, q, "multi: solve");
-        }));
-    TRY((launch_groups<kTeam, StTail<true>, 0>(p->t32_64, p->lds_tail32, s, "multi: tail")));
-  }
-  if (p->t64.ngroups) {
-    TRY((launch_groups<kTeam, StFusedStep, 0>(p->t64, p->lds_step, s, "multi: step")));
-    TRY((launch_groups<kTeam, StTail<true>, 0>(p->t64, p->lds_tail64, s, "multi: tail")));
-  }
-  return MJH_OK;
+  plan.t32.blocks = (int)b32;
+  plan.t32_64.blocks = (int)b32_64;
+  plan.t64.blocks = (int)b64;
+  return multi_launches(plan, s, false);
 }
 int mjh_multi_destroy(mjhMulti* p) {
   if (!p) return MJH_OK;

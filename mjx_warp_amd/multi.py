@@ -8,7 +8,11 @@ and advances every world of every group by one fused step per `step()`.
 In the "grouped" mode the library runs one launch per stage over all groups
 (`mjh_multi_step`: each workgroup finds its group in a device table), so
 many small groups cost the launches of one batch; in the "per_group" mode
-it issues one `mjh_step_fused` per group.  `autotune=True` times both modes
+it issues one `mjh_step_fused` per group; the "static" mode runs the
+grouped launches through `mjh_multi_step_static`, which keeps the group
+table in a fixed device array of the library and allocates nothing per
+call (for callers that cannot allocate, e.g. under graph capture; at most
+64 groups, and calls on one GPU must not overlap).  `autotune=True` times the grouped and per-group modes
 on the groups' own state (restored afterwards) and keeps the faster one,
 remembered per group signature (each group's sizes and world count) for
 the rest of the process.
@@ -42,7 +46,7 @@ class MultiStep:
   def __init__(self, groups, autotune: bool = False, mode: str = "grouped"):
     if not groups:
       raise ValueError("MultiStep needs at least one (Model, Data) group")
-    if mode not in ("grouped", "per_group"):
+    if mode not in ("grouped", "per_group", "static"):
       raise ValueError(f"unknown mode {mode!r}")
     self.groups = [(m, d) for m, d in groups]
     self.mode = mode
@@ -63,12 +67,15 @@ class MultiStep:
     n = len(self.groups)
     M = (type(ms[0]) * n)(*ms)
     D = (type(ds[0]) * n)(*ds)
+    self._arrays = (M, D)
+    self._structs = ds
+    if self.mode == "static":
+      return  # no plan: the library's own table
     plan = ctypes.c_void_p()
     rc = lib.mjh_multi_create(n, M, D, ctypes.byref(plan))
     if rc != _native.MJH_OK:
       _native._raise(rc, "mjh_multi_create")
     self._plan = plan
-    self._structs = ds
 
   def _stale(self) -> bool:
     return any(d._abi is not s for (_, d), s in zip(self.groups, self._structs))
@@ -82,6 +89,11 @@ class MultiStep:
       rc = _native.lib().mjh_multi_step(self._plan, _native._stream())
       if rc != _native.MJH_OK:
         _native._raise(rc, "mjh_multi_step")
+    elif self.mode == "static":
+      M, D = self._arrays
+      rc = _native.lib().mjh_multi_step_static(len(self.groups), M, D, _native._stream())
+      if rc != _native.MJH_OK:
+        _native._raise(rc, "mjh_multi_step_static")
     else:
       for m, d in self.groups:
         _native.call("mjh_step_fused", m, d)
