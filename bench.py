@@ -59,20 +59,39 @@ def algorithmic_bytes(mjm, ncon_w: float, nefc_w: float) -> float:
 PROFILE = os.path.join(ROOT, "profiles", "step_humanoid_b8192_newton10.md")
 
 
+def kernel_sources_sha() -> str:
+  """sha256 over the HIP sources (csrc/*.hpp, *.hip in name order): the
+  profile summary records it, so a measured figure is used only for the
+  kernels it was measured on."""
+  import glob
+  import hashlib
+
+  h = hashlib.sha256()
+  for path in sorted(glob.glob(os.path.join(ROOT, "mjx_warp_amd", "csrc", "*.hpp"))
+                     + glob.glob(os.path.join(ROOT, "mjx_warp_amd", "csrc", "*.hip"))):
+    with open(path, "rb") as f:
+      h.update(os.path.basename(path).encode() + b"\0" + f.read())
+  return h.hexdigest()[:16]
+
+
 def measured_traffic(args):
   """HBM bytes of one fused step measured with rocprofv3 FETCH_SIZE /
   WRITE_SIZE (scaled by a plain copy's counters) for the default workload,
-  as recorded in the profile summary; None for other configurations."""
+  as recorded in the profile summary; None for other configurations or
+  when the profile was taken on other kernel sources."""
   if (args.batch, args.iterations, args.ls_iterations) != (8192, 10, 20):
     return None
+  traffic, sha = None, None
   try:
     with open(PROFILE) as f:
       for line in f:
         if line.startswith("fused step traffic per launch"):
-          return float(line.rsplit(":", 1)[1])
+          traffic = float(line.rsplit(":", 1)[1])
+        if line.startswith("kernel sources sha256"):
+          sha = line.rsplit(":", 1)[1].strip()
   except OSError:
     pass
-  return None
+  return traffic if sha == kernel_sources_sha() else None
 
 
 def setup(args, device="cuda"):
@@ -90,7 +109,10 @@ def setup(args, device="cuda"):
   return mjm, m, d
 
 
-def time_form(args, fused: bool):
+def time_form(args, fused: bool, preroll: int = 0):
+  """Times args.steps steps after args.warmup warm-up steps and `preroll`
+  more untimed steps (1000 take the humanoids from the squat keyframe to
+  lying on the floor, 100-150 constraint rows each)."""
   import torch
 
   import mjx_warp_amd as mjw
@@ -111,7 +133,7 @@ def time_form(args, fused: bool):
     d.ctrl.copy_(u.mul_(2.0).sub_(1.0))
     stepfn(m, d)
 
-  for _ in range(args.warmup):
+  for _ in range(args.warmup + preroll):
     one_step()
   torch.cuda.synchronize()
   stream = torch.cuda.current_stream()
@@ -310,7 +332,10 @@ def cpu_baseline(args, target_s: float = 15.0):
   sys.path.insert(0, os.path.join(ROOT, "tests"))
   import helpers
 
-  cores = min(16, len(os.sched_getaffinity(0)))
+  # the GPU box's CPU share is 16 cores per GPU: the oracle's OpenMP pool
+  # is sized to it whatever the host shows
+  visible = len(os.sched_getaffinity(0))
+  cores = min(16, visible)
   B = min(args.batch, 1024)
   mjm, m, d = setup(argparse.Namespace(**{**vars(args), "batch": B}), device="cpu")
   hm = helpers.host_model(m, np.float32)
@@ -338,7 +363,9 @@ def cpu_baseline(args, target_s: float = 15.0):
     pass
   return dict(value=B * nstep / dt, unit="world-steps/s", cores=cores, kind="port",
               sample=f"{B} humanoid worlds x {nstep} steps of the oracle (FP32, OpenMP over worlds) "
-                     f"after 1 warm-up step, on {cpu or 'host'}")
+                     f"after 1 warm-up step, on {cpu or 'host'}",
+              threads_used=cores, host_cores_visible=visible, cpu_model=cpu,
+              threads_note="the OpenMP pool is the GPU box's CPU share of 16 cores per GPU")
 
 
 def copy_probe(mib: int, reps: int = 5):
@@ -368,6 +395,8 @@ def main():
   ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint rollout")
   ap.add_argument("--no-rollout-graph", action="store_true", help="skip the graph-replayed rollout")
   ap.add_argument("--no-hetero", action="store_true", help="skip the heterogeneous (many-model) batch")
+  ap.add_argument("--no-fallen", action="store_true", help="skip the fallen-humanoid leg")
+  ap.add_argument("--fallen-preroll", type=int, default=1000, help="untimed steps before the fallen leg")
   ap.add_argument("--graph", action="store_true", help="replay a HIP graph of one captured step")
   ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                   help="only run a plain copy of MIB MiB (counter calibration) and exit")
@@ -391,6 +420,22 @@ def main():
   if not args.no_plain:
     _, plain = time_form(args, fused=False)
     log(f"plain: {plain['value']:.4g} world-steps/s, {plain['ms_per_step']:.3f} ms/step")
+  fallen = None
+  if not args.no_fallen:
+    # the same workload after 1000 untimed steps: the humanoids have fallen
+    # and lie on the floor with 100-150 rows each (the reference benchmark's
+    # 1000-step horizon, testspeed.py:42)
+    _, ff = time_form(args, fused=True, preroll=args.fallen_preroll)
+    fallen = {"preroll_steps": args.fallen_preroll, "value": ff["value"], "ms_per_step": ff["ms_per_step"],
+              "kernel_ms": ff["kernel_ms"], "mean_ncon_per_world": ff["ncon_w"],
+              "mean_nefc_per_world": ff["nefc_w"], "state_finite": ff["finite"]}
+    if not args.no_plain:
+      _, fp = time_form(args, fused=False, preroll=args.fallen_preroll)
+      fallen["plain_value"] = fp["value"]
+      fallen["plain_ms_per_step"] = fp["ms_per_step"]
+      fallen["fused_over_plain"] = ff["value"] / fp["value"]
+    log(f"fallen (after {args.fallen_preroll} steps): fused {ff['value']:.4g} world-steps/s, "
+        f"{ff['nefc_w']:.1f} rows per world" + (f", plain {fallen['plain_value']:.4g}" if "plain_value" in fallen else ""))
   rgraph = None
   if not args.no_rollout_graph:
     rgraph = time_rollout_graph(args)
@@ -431,7 +476,10 @@ def main():
                "launch": "hip graph replay" if args.graph else "eager"},
     "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
-                 "kernel": "fused step: counter reset, forward launch, solve+integrate launch, tail",
+                 "kernel": "fused step: description write (with the counter reset), forward launch, "
+                           "solve+integrate launch with the big-world launch beside it, tail",
+                 "limiter": "latency and instruction issue, not HBM: the step runs at a few percent of the "
+                            "HBM peak (dependent per-world chains, pool reservations, the solver's VALU work)",
                  "algorithmic_bytes_per_world_step": bytes_ws, "worlds_per_launch": args.batch,
                  "kernel_ms": fused["kernel_ms"]},
     "cpu_baseline": base,
@@ -439,7 +487,9 @@ def main():
     "mean_nefc_per_world": fused["nefc_w"],
     "state_finite": fused["finite"],
     "plain_step": None if plain is None else {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
-                                              "kernel_ms": plain["kernel_ms"]},
+                                              "kernel_ms": plain["kernel_ms"],
+                                              "fused_over_plain": fused["value"] / plain["value"]},
+    "fallen": fallen,
     "rollout_graph": rgraph,
     "forward_adjoint": adj,
     "heterogeneous": het,

@@ -165,6 +165,9 @@ def data_struct(m, d):
   _, DataStruct = structs()
   _, arrays = layout()["data"]
   sizes = data_sizes(m, d)
+  if sizes["ndesc"] not in (0, step_desc_words()):
+    # the step writes a whole description into a non-empty step_desc
+    raise ValueError(f"step_desc must hold 0 or {step_desc_words()} words, not {sizes['ndesc']}")
   s = DataStruct()
   for k in ("nworld", "nconmax", "njmax", "qM_rows", "qM_cols", "qMd_n"):
     setattr(s, k, sizes[k])

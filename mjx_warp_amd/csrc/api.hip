@@ -62,6 +62,140 @@ __global__ void __launch_bounds__(64) k_xfrc(mjhModel m, mjhData d, float* qfrc)
   xfrc_accumulate(m, d, qfrc, t);
 }
 
+// ---- constraint-row pools laid out by the caller ----
+// The reference appends rows to its pool with atomics (constraint.py:119,
+// 162), so a world's rows may sit anywhere in [0, nefc), tagged by
+// efc_worldid (-1 for unused rows); the solver kernels here read a world's
+// rows as the block world_efc[w] = (first, count).  mjh_solve derives the
+// blocks from efc_worldid before solving: a world whose rows are contiguous
+// gets (first, count); a world whose rows are interleaved with others' is
+// gathered, in row order, into the pool's unused capacity [nefc, njmax),
+// solved there, and its efc_force / efc_active / efc_Jaref are scattered
+// back to its own rows afterwards.  world_flags bit 1 marks gathered
+// worlds; the bits above bit 1 serve as a per-world counter meanwhile.
+constexpr int kFlagGathered = 2;
+constexpr int kFlagCount = 4;
+
+MJH_DEV int pool_rows(const mjhData& d) { return min(max(d.nefc[0], 0), d.njmax); }
+
+// per world: clear the block and the counter bits (keeps the flags)
+__global__ void k_blocks_zero(mjhData d) {
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < d.nworld; w += gridDim.x * blockDim.x) {
+    d.world_efc[2 * w] = 0;
+    d.world_efc[2 * w + 1] = 0;
+    d.world_flags[w] &= kFlagDeferred;
+  }
+}
+// per row: count each world's rows and its runs of consecutive rows
+__global__ void k_blocks_count(mjhData d) {
+  const int n = pool_rows(d);
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const int w = d.efc_worldid[r];
+    if (w < 0 || w >= d.nworld) continue;
+    atomicAdd(d.world_efc + 2 * w + 1, 1);
+    if (r == 0 || d.efc_worldid[r - 1] != w) {
+      atomicAdd(d.world_flags + w, kFlagCount);
+      d.world_efc[2 * w] = r;  // the run's first row (the only run of a contiguous world)
+    }
+  }
+}
+// one workgroup of 1024: worlds with more than one run are gathered; their
+// blocks are laid out back to back after the pool's rows (an exclusive scan
+// over worlds in chunks of 1024), as far as the capacity goes
+__global__ void __launch_bounds__(1024) k_blocks_place(mjhData d) {
+  __shared__ int part[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = pool_rows(d);
+  __syncthreads();
+  for (int w0 = 0; w0 < d.nworld; w0 += 1024) {
+    const int w = w0 + (int)threadIdx.x;
+    int f = 0, n = 0;
+    bool gather = false;
+    if (w < d.nworld) {
+      f = d.world_flags[w];
+      gather = f / kFlagCount > 1;
+      n = gather ? d.world_efc[2 * w + 1] : 0;
+    }
+    part[threadIdx.x] = n;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+      const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+      __syncthreads();
+      part[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int start = carry + part[threadIdx.x] - n;
+    if (w < d.nworld) {
+      f &= kFlagDeferred;
+      if (gather) {
+        const int fit = max(0, min(n, d.njmax - start));
+        if (fit < n) atomicAdd(d.overflow + 2, n - fit);
+        d.world_efc[2 * w] = start;
+        d.world_efc[2 * w + 1] = fit;
+        f |= kFlagGathered;
+      }
+      d.world_flags[w] = f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += part[1023];
+    __syncthreads();
+  }
+}
+// per row of a gathered world: its source index into the world's new
+// block (efc_worldid of the unused capacity), in arrival order
+__global__ void k_blocks_slots(mjhData d) {
+  const int n = pool_rows(d);
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const int w = d.efc_worldid[r];
+    if (w < 0 || w >= d.nworld || !(d.world_flags[w] & kFlagGathered)) continue;
+    const int slot = atomicAdd(d.world_flags + w, kFlagCount) / kFlagCount;
+    if (slot < d.world_efc[2 * w + 1]) d.efc_active[d.world_efc[2 * w] + slot] = r;
+  }
+}
+// one wavefront per gathered world: the source rows in ascending order
+// (rank sort from efc_active into efc_worldid of the new block), then the
+// rows' J, D and aref copied in that order; efc_worldid of the new block
+// keeps the source rows for the scatter
+__global__ void __launch_bounds__(64) k_blocks_gather(mjhData d, int nv) {
+  const int w = blockIdx.x;
+  if (w >= d.nworld || !(d.world_flags[w] & kFlagGathered)) return;
+  const int b = d.world_efc[2 * w], n = d.world_efc[2 * w + 1];
+  const int* src = d.efc_active + b;
+  for (int i = threadIdx.x; i < n; i += 64) {
+    const int v = src[i];
+    int rank = 0;
+    for (int j = 0; j < n; j++) rank += src[j] < v;
+    d.efc_worldid[b + rank] = v;
+  }
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    const int r = d.efc_worldid[b + k];
+    for (int i = threadIdx.x; i < nv; i += 64) d.efc_J[(size_t)(b + k) * nv + i] = d.efc_J[(size_t)r * nv + i];
+    if (threadIdx.x == 0) {
+      d.efc_D[b + k] = d.efc_D[r];
+      d.efc_aref[b + k] = d.efc_aref[r];
+      d.efc_pos[b + k] = d.efc_pos[r];
+      d.efc_margin[b + k] = d.efc_margin[r];
+    }
+  }
+  if (threadIdx.x == 0) d.world_flags[w] = (d.world_flags[w] & kFlagDeferred) | kFlagGathered;
+}
+// after the solve: the gathered rows' results back to the caller's rows
+__global__ void __launch_bounds__(64) k_blocks_scatter(mjhData d) {
+  const int w = blockIdx.x;
+  if (w >= d.nworld || !(d.world_flags[w] & kFlagGathered)) return;
+  const int b = d.world_efc[2 * w], n = d.world_efc[2 * w + 1];
+  for (int k = threadIdx.x; k < n; k += 64) {
+    const int r = d.efc_worldid[b + k];
+    d.efc_force[r] = d.efc_force[b + k];
+    d.efc_active[r] = d.efc_active[b + k];
+    d.efc_Jaref[r] = d.efc_Jaref[b + k];
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += 64) d.efc_worldid[b + k] = w;
+  if (threadIdx.x == 0) d.world_flags[w] &= ~kFlagGathered;
+}
+
 // transition derivatives and the adjoint recursion
 __global__ void __launch_bounds__(64) k_fd_perturb(mjhModel m, mjhData d, mjhData fd, float eps, int centered) {
   Team<kTeam> t(fd.nworld);
@@ -237,6 +371,7 @@ int launch_team(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, co
                 const Tune* tune = nullptr) {
   if (d.nworld <= 0) return MJH_OK;
   constexpr int wpb = 64 / TEAM;
+  if (wpb > 1 && ldsw % 4 != 0) return fail(MJH_ERR_INVALID, std::string(name) + ": LDS stride not 16-byte aligned");
   const int grid = (d.nworld + wpb - 1) / wpb;
   const size_t bytes = (size_t)ldsw * wpb * sizeof(float);
   if (bytes > 65536)
@@ -246,11 +381,18 @@ int launch_team(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, co
   return check_launch(name);
 }
 
+// Per-world LDS stride of a 32-lane launch: at least kRegLdsMin floats and
+// a multiple of 4, so every world's slot (and the float4 buffers stages put
+// at its start) is 16-byte aligned.
+int reg_stride(int ldsw) { return ((ldsw > kRegLdsMin ? ldsw : kRegLdsMin) + 3) & ~3; }
+// the fused forward launch of 32-lane teams
+int lds_forward_reg(const mjhModel& m) { return reg_stride(lds_forward(m)); }
+
 // A stage on the model's team size: 32-lane teams (two worlds per
 // wavefront) when the register path applies, else 64-lane teams.
 template <class S>
 int launch(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, const char* name) {
-  if (reg_path(m)) return launch_team<S, kRegTeam>(m, d, s, ((ldsw > kRegLdsMin ? ldsw : kRegLdsMin) + 3) & ~3, name);
+  if (reg_path(m)) return launch_team<S, kRegTeam>(m, d, s, reg_stride(ldsw), name);
   return launch_team<S, kTeam>(m, d, s, ldsw, name);
 }
 
@@ -331,7 +473,7 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
   if (d.step_desc && d.nworld > 0) return do_step_fused_desc(m, d, s);
   TRY(reset(d, s, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3));
   if (reg_path(m)) {
-    TRY((launch_team<StFusedForward, kRegTeam>(m, d, s, lds_forward(m), "step_fused_forward")));
+    TRY((launch_team<StFusedForward, kRegTeam>(m, d, s, lds_forward_reg(m), "step_fused_forward")));
     const Tune tt = tune_tail_reg(m);
     TRY(beside(
         s,
@@ -397,6 +539,7 @@ template <int TEAM, class S, int TI>
 int launch_groups(const GroupTable& t, int ldsw, hipStream_t s, const char* name) {
   if (t.ngroups == 0 || t.blocks == 0) return MJH_OK;
   constexpr int wpb = 64 / TEAM;
+  if (wpb > 1 && ldsw % 4 != 0) return fail(MJH_ERR_INVALID, std::string(name) + ": LDS stride not 16-byte aligned");
   const size_t bytes = (size_t)ldsw * wpb * sizeof(float);
   hipLaunchKernelGGL((k_stage_groups<TEAM, S, TI>), dim3(t.blocks), dim3(64), bytes, s, t.dev, t.ngroups, ldsw);
   return check_launch(name);
@@ -457,7 +600,7 @@ int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
   g.tune[1] = reg_path(m) ? tune_tail_reg(m) : g.tune[0];
   GroupDesc* dev = reinterpret_cast<GroupDesc*>(d.step_desc);
   const bool reg = reg_path(m);
-  const int l_fwd = lds_forward(m), l_solve = lds_solve_reg(m), l_big = lds_tail_reg(m), l_step = lds_step(m),
+  const int l_fwd = lds_forward_reg(m), l_solve = lds_solve_reg(m), l_big = lds_tail_reg(m), l_step = lds_step(m),
             l_tail = lds_linalg(m);
   const size_t need = reg ? (size_t)max_of(max_of(2 * l_fwd, 2 * l_solve), max_of(2 * l_big, l_tail))
                           : (size_t)max_of(l_step, l_tail);
@@ -500,7 +643,7 @@ int plan_group(int i, const mjhModel& m, const mjhData& d, mjhMulti* plan, Group
   *reg = reg_path(m);
   if (*reg) {
     gd->tune[1] = tune_tail_reg(m);
-    plan->lds_fwd = max_of(plan->lds_fwd, lds_forward(m));
+    plan->lds_fwd = max_of(plan->lds_fwd, lds_forward_reg(m));
     plan->lds_solve = max_of(plan->lds_solve, lds_solve_reg(m));
     plan->lds_big = max_of(plan->lds_big, lds_tail_reg(m));
     plan->lds_tail32 = max_of(plan->lds_tail32, lds_linalg(m));
@@ -617,7 +760,23 @@ STAGE_ENTRY(mjh_fwd_actuation, StFwdActuation, 0)
 STAGE_ENTRY(mjh_fwd_acceleration, StFwdAcceleration, lds_linalg(*m))
 
 int mjh_solve(const mjhModel* m, const mjhData* d, mjhStream stream) {
-  return do_solve(*m, *d, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  if (d->nworld > 0 && d->njmax > 0) {
+    // each world's rows from efc_worldid (the pool may be laid out by the caller)
+    const int wg = (d->nworld + 255) / 256, rg = (d->njmax + 255) / 256;
+    hipLaunchKernelGGL(k_blocks_zero, dim3(wg), dim3(256), 0, s, *d);
+    hipLaunchKernelGGL(k_blocks_count, dim3(rg < 4096 ? rg : 4096), dim3(256), 0, s, *d);
+    hipLaunchKernelGGL(k_blocks_place, dim3(1), dim3(1024), 0, s, *d);
+    hipLaunchKernelGGL(k_blocks_slots, dim3(rg < 4096 ? rg : 4096), dim3(256), 0, s, *d);
+    hipLaunchKernelGGL(k_blocks_gather, dim3(d->nworld), dim3(64), 0, s, *d, m->nv);
+    TRY(check_launch("solve: row blocks"));
+  }
+  TRY(do_solve(*m, *d, s));
+  if (d->nworld > 0 && d->njmax > 0) {
+    hipLaunchKernelGGL(k_blocks_scatter, dim3(d->nworld), dim3(64), 0, s, *d);
+    TRY(check_launch("solve: scatter"));
+  }
+  return MJH_OK;
 }
 int mjh_nxn_broadphase(const mjhModel* m, const mjhData* d, mjhStream stream) {
   TRY(reset(*d, (hipStream_t)stream, 1));

@@ -378,8 +378,14 @@ struct ContactSink {
   int base, cap, k, world;
   bool write;
   int g1, g2;
+  // counting the contacts that become constraint rows (constraint.py:
+  // 155-156): dim 3 and inside the include margin; dim 0 counts none
+  int dim = 0;
+  float incl = 0.0f;
+  int nact = 0;
   MJH_DEV void emit(float dist, V3 pos, const M33& frame, float margin) {
     if (!((dist - margin) < 0.0f)) return;
+    if (dim == 3 && dist - incl < 0.0f) nact++;
     const int idx = base + k++;
     if (!write || idx >= cap) return;
     d->contact_dist[idx] = dist;
@@ -498,11 +504,29 @@ MJH_DEV void narrowphase_pair(const mjhModel& m, const float* gxpos, const float
   // sphere-capsule and other pairs produce no contacts in the reference
 }
 
+// collision_driver.py:62-65: pairs are ordered by geom type
+MJH_DEV void type_order(const mjhModel& m, int* g1, int* g2) {
+  if (m.geom_type[*g1] > m.geom_type[*g2]) {
+    const int tmp = *g1;
+    *g1 = *g2;
+    *g2 = tmp;
+  }
+}
+// contact dimension and include margin of the pair's contacts, as
+// contact_params mixes them (collision_driver.py:304-347)
+MJH_DEV int pair_condim(const mjhModel& m, int g1, int g2) {
+  const int p1 = m.geom_priority[g1], p2 = m.geom_priority[g2];
+  const int c1 = m.geom_condim[g1], c2 = m.geom_condim[g2];
+  return (p1 == p2) ? max(c1, c2) : (p1 > p2 ? c1 : c2);
+}
+MJH_DEV float pair_includemargin(const mjhModel& m, int g1, int g2) {
+  return fmaxf(m.geom_margin[g1], m.geom_margin[g2]) - fmaxf(m.geom_gap[g1], m.geom_gap[g2]);
+}
+
 // collision_driver.py:304-347 contact parameter mixing of the contacts
 // [idx, idx + n) of the geom pair (g1, g2)
 MJH_DEV void contact_params(const mjhModel& m, const mjhData& d, int g1, int g2, int idx, int n) {
-  const float margin = fmaxf(m.geom_margin[g1], m.geom_margin[g2]);
-  const float gap = fmaxf(m.geom_gap[g1], m.geom_gap[g2]);
+  const float incl = pair_includemargin(m, g1, g2);
   const float sm1 = m.geom_solmix[g1], sm2 = m.geom_solmix[g2];
   float mix = sm1 / (sm1 + sm2);
   if (sm1 < MJH_MINVAL && sm2 < MJH_MINVAL) mix = 0.5f;
@@ -510,8 +534,7 @@ MJH_DEV void contact_params(const mjhModel& m, const mjhData& d, int g1, int g2,
   else if (sm1 >= MJH_MINVAL && sm2 < MJH_MINVAL) mix = 1.0f;
   const int p1 = m.geom_priority[g1], p2 = m.geom_priority[g2];
   if (p1 != p2) mix = p1 > p2 ? 1.0f : 0.0f;
-  const int c1 = m.geom_condim[g1], c2 = m.geom_condim[g2];
-  const int dim = (p1 == p2) ? max(c1, c2) : (p1 > p2 ? c1 : c2);
+  const int dim = pair_condim(m, g1, g2);
   const float* sr1 = m.geom_solref + 2 * g1;
   const float* sr2 = m.geom_solref + 2 * g2;
   float sr[2];
@@ -534,7 +557,7 @@ MJH_DEV void contact_params(const mjhModel& m, const mjhData& d, int g1, int g2,
     d.contact_dim[c] = dim;
     d.contact_solref[2 * c] = sr[0];
     d.contact_solref[2 * c + 1] = sr[1];
-    d.contact_includemargin[c] = margin - gap;
+    d.contact_includemargin[c] = incl;
     float* fr = d.contact_friction + 5 * c;
     fr[0] = fa;
     fr[1] = fa;

@@ -308,6 +308,32 @@ MJH_DEV int team_reserve(int* counter, int total) {
   return (TEAM == 32 && (threadIdx.x & 32)) ? base + t0 : base;
 }
 
+// Three blocks of three pools in one round trip: the wavefront's first lane
+// issues the three atomics back to back and waits once; each team gets its
+// bases in *b0..*b2 (as team_reserve).  Every lane of the wavefront must
+// call it.
+template <int TEAM>
+MJH_DEV void team_reserve3(int* c0, int n0, int* c1, int n1, int* c2, int n2, int* b0, int* b1, int* b2) {
+  const int a0 = __builtin_amdgcn_readlane(n0, 0), a1 = __builtin_amdgcn_readlane(n1, 0),
+            a2 = __builtin_amdgcn_readlane(n2, 0);
+  const int z0 = TEAM == 64 ? 0 : __builtin_amdgcn_readlane(n0, 32);
+  const int z1 = TEAM == 64 ? 0 : __builtin_amdgcn_readlane(n1, 32);
+  const int z2 = TEAM == 64 ? 0 : __builtin_amdgcn_readlane(n2, 32);
+  int r0 = 0, r1 = 0, r2 = 0;
+  if ((threadIdx.x & 63) == 0) {
+    r0 = atomicAdd(c0, a0 + z0);
+    r1 = atomicAdd(c1, a1 + z1);
+    r2 = atomicAdd(c2, a2 + z2);
+  }
+  r0 = __builtin_amdgcn_readlane(r0, 0);
+  r1 = __builtin_amdgcn_readlane(r1, 0);
+  r2 = __builtin_amdgcn_readlane(r2, 0);
+  const bool upper = TEAM == 32 && (threadIdx.x & 32);
+  *b0 = upper ? r0 + a0 : r0;
+  *b1 = upper ? r1 + a1 : r1;
+  *b2 = upper ? r2 + a2 : r2;
+}
+
 // This lane's team as a mask over the wavefront's lanes.
 template <int TEAM>
 MJH_DEV unsigned long long team_lanes() {

@@ -5,6 +5,7 @@
 // order, the same per-world order as the reference's two launches.
 #pragma once
 
+#include "collision.hpp"
 #include "common.hpp"
 
 namespace mjh {
@@ -56,10 +57,13 @@ __host__ __device__ constexpr int lds_constraint_floats() {
   return 29 * TEAM;
 }
 
-// constraint.py:213-240 make_constraint (nefc is reset by the caller)
+// The rows of one world into the block [base, base + total) of the pool
+// (reserved by the caller; total counts nlim limit rows first): limit rows
+// in joint order, then four rows per active contact in contact order, as
+// many as the block and the pool hold.
 template <int TEAM>
-MJH_DEV void make_constraint(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
-  if (m.opt.disableflags & mjhDSBL_CONSTRAINT) return;
+MJH_DEV void make_constraint_rows(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t, int base,
+                                  int total, int nlim) {
   const int w = t.world, nv = m.nv;
   const bool refsafe = !(m.opt.disableflags & mjhDSBL_REFSAFE);
   const bool do_limit = !(m.opt.disableflags & mjhDSBL_LIMIT) && m.nlimited > 0;
@@ -71,23 +75,6 @@ MJH_DEV void make_constraint(const mjhModel& m, const mjhData& d, float* lds, co
     cadr = d.world_con[2 * w];
     ncon = d.world_con[2 * w + 1];
   }
-  // count rows
-  int nlim = 0, ncact = 0;
-  if (t.valid) {
-    if (do_limit)
-      for (int k = t.lane; k < m.nlimited; k += TEAM) {
-        float pos, J;
-        nlim += limit_active(m, qpos, m.jnt_limited_slide_hinge_adr[k], &pos, &J) ? 1 : 0;
-      }
-    for (int c = t.lane; c < ncon; c += TEAM) {
-      const int ci = cadr + c;
-      if (d.contact_dim[ci] == 3 && d.contact_dist[ci] - d.contact_includemargin[ci] < 0.0f) ncact++;
-    }
-  }
-  nlim = team_sum_int<TEAM>(nlim);
-  ncact = team_sum_int<TEAM>(ncact);
-  const int total = nlim + 4 * ncact;
-  const int base = team_reserve<TEAM>(d.nefc, t.valid ? total : 0);
   const int cap = d.njmax;
   const int nwritten = total <= 0 ? 0 : max(0, min(total, cap - base));
   if (!t.valid) return;
@@ -225,6 +212,160 @@ MJH_DEV void make_constraint(const mjhModel& m, const mjhData& d, float* lds, co
     sync();
     rrun += 4 * nact;
   }
+}
+
+// constraint.py:213-240 make_constraint (nefc is reset by the caller):
+// count the world's rows, reserve its block, write the rows
+template <int TEAM>
+MJH_DEV void make_constraint(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
+  if (m.opt.disableflags & mjhDSBL_CONSTRAINT) return;
+  const int w = t.world;
+  const bool do_limit = !(m.opt.disableflags & mjhDSBL_LIMIT) && m.nlimited > 0;
+  const bool do_contact = !(m.opt.disableflags & mjhDSBL_CONTACT) && m.opt.cone == mjhCONE_PYRAMIDAL;
+  const float* qpos = WPTR(d.qpos, w, m.nq);
+  int cadr = 0, ncon = 0;
+  if (t.valid && do_contact) {
+    cadr = d.world_con[2 * w];
+    ncon = d.world_con[2 * w + 1];
+  }
+  int nlim = 0, ncact = 0;
+  if (t.valid) {
+    if (do_limit)
+      for (int k = t.lane; k < m.nlimited; k += TEAM) {
+        float pos, J;
+        nlim += limit_active(m, qpos, m.jnt_limited_slide_hinge_adr[k], &pos, &J) ? 1 : 0;
+      }
+    for (int c = t.lane; c < ncon; c += TEAM) {
+      const int ci = cadr + c;
+      if (d.contact_dim[ci] == 3 && d.contact_dist[ci] - d.contact_includemargin[ci] < 0.0f) ncact++;
+    }
+  }
+  nlim = team_sum_int<TEAM>(nlim);
+  ncact = team_sum_int<TEAM>(ncact);
+  const int total = nlim + 4 * ncact;
+  const int base = team_reserve<TEAM>(d.nefc, t.valid ? total : 0);
+  make_constraint_rows(m, d, lds, t, base, total, nlim);
+}
+
+// The fused forward's collision and row reservation for models the nxn
+// broadphase serves (ngeom <= 100; collision_driver.py:458-547 and the
+// counting half of constraint.py:213-240).  A counting pass tests the
+// candidate pairs and runs each hit pair's narrowphase without writing:
+// its contacts, and those of them that become rows (condim 3 inside the
+// include margin); the limit rows are counted beside.  The world's three
+// pool blocks (pairs, contacts, rows) are then reserved by three atomics
+// in flight together -- one round trip where collision() and
+// make_constraint() take three -- and a write pass stores the pairs and
+// contacts (with their parameters) exactly as nxn_broadphase and
+// narrowphase do: candidate order, type-ordered pairs, contacts only of
+// pairs inside the pair pool.  Returns the row block (*rbase, *rtotal,
+// limit rows first, *rnlim of them) for make_constraint_rows.
+template <int TEAM>
+MJH_DEV void collide_and_reserve(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t, int* rbase,
+                                 int* rtotal, int* rnlim) {
+  const int w = t.world, ng = m.ngeom;
+  const bool fp = !(m.opt.disableflags & mjhDSBL_FILTERPARENT);
+  const float* gxpos = WPTR(d.geom_xpos, w, ng * 3);
+  const float* gxmat = WPTR(d.geom_xmat, w, ng * 9);
+  const bool table = ng <= MJH_NXN_TABLE_MAX;
+  const int ncand = table ? m.nnxn : ng * (ng - 1) / 2;
+  const bool rows = !(m.opt.disableflags & mjhDSBL_CONSTRAINT);
+  const bool do_limit = rows && !(m.opt.disableflags & mjhDSBL_LIMIT) && m.nlimited > 0;
+  const bool do_crow = rows && !(m.opt.disableflags & mjhDSBL_CONTACT) && m.opt.cone == mjhCONE_PYRAMIDAL;
+  if (t.valid) nxn_stage_geoms(m, gxpos, gxmat, lds, !table, t);
+  sync();
+  // counting pass; the first 21 rounds keep each lane's hit (bit i) and its
+  // pair's contact count (3 bits at 3 i: a pair yields at most 4 contacts)
+  int npair = 0, ncon = 0, ncact = 0, nlim = 0;
+  unsigned hits = 0;
+  unsigned long long cnts = 0;
+  if (t.valid) {
+    for (int p = t.lane, i = 0; p < ncand; p += TEAM, i++) {
+      int g1, g2;
+      if (!nxn_candidate(m, lds, p, table, fp, true, &g1, &g2)) continue;
+      npair++;
+      if (i < 32) hits |= 1u << i;
+      type_order(m, &g1, &g2);
+      ContactSink s{&d, 0, 0, 0, w, false, 0, 0};
+      if (do_crow) {
+        s.dim = pair_condim(m, g1, g2);
+        s.incl = pair_includemargin(m, g1, g2);
+      }
+      narrowphase_pair(m, gxpos, gxmat, g1, g2, s);
+      ncon += s.k;
+      ncact += s.nact;
+      if (i < 21) cnts |= (unsigned long long)s.k << (3 * i);
+    }
+    if (do_limit) {
+      const float* qpos = WPTR(d.qpos, w, m.nq);
+      for (int k = t.lane; k < m.nlimited; k += TEAM) {
+        float pos, J;
+        nlim += limit_active(m, qpos, m.jnt_limited_slide_hinge_adr[k], &pos, &J) ? 1 : 0;
+      }
+    }
+  }
+  npair = team_sum_int<TEAM>(npair);
+  ncon = team_sum_int<TEAM>(ncon);
+  ncact = team_sum_int<TEAM>(ncact);
+  nlim = team_sum_int<TEAM>(nlim);
+  const int nrow = rows ? nlim + 4 * ncact : 0;
+  int pbase, cbase, ebase;
+  team_reserve3<TEAM>(d.ncollision, t.valid ? npair : 0, d.ncon, t.valid ? ncon : 0, d.nefc, t.valid ? nrow : 0,
+                      &pbase, &cbase, &ebase);
+  *rbase = ebase;
+  *rtotal = nrow;
+  *rnlim = nlim;
+  if (!t.valid) return;
+  const int cap = d.nconmax;
+  const int npw = npair <= 0 ? 0 : max(0, min(npair, cap - pbase));
+  const int ncw = ncon <= 0 ? 0 : max(0, min(ncon, cap - cbase));
+  if (t.lane == 0) {
+    d.world_pair[2 * w] = pbase;
+    d.world_pair[2 * w + 1] = npw;
+    if (pbase + npair > cap) atomicAdd(d.overflow + 0, min(npair, pbase + npair - cap));
+    d.world_con[2 * w] = cbase;
+    d.world_con[2 * w + 1] = ncw;
+    if (cbase + ncon > cap) atomicAdd(d.overflow + 1, min(ncon, cbase + ncon - cap));
+  }
+  // write pass
+  int kp = 0, kc = 0;
+  for (int p0 = 0, i = 0; p0 < ncand; p0 += TEAM, i++) {
+    const int p = p0 + t.lane;
+    int g1 = 0, g2 = 0;
+    bool ok = false;
+    if (p < ncand) {
+      ok = nxn_candidate(m, lds, p, table, fp, i >= 32, &g1, &g2);
+      if (i < 32) ok = ((hits >> i) & 1u) != 0;
+    }
+    int tot;
+    const int off = team_exscan_bits<TEAM, 1>(ok ? 1 : 0, &tot);
+    int c = 0;
+    if (ok) {
+      PairWriter<TEAM> pw{m, d, pbase, cap, kp + off};
+      pw.put(g1, g2, w);
+      type_order(m, &g1, &g2);
+      if (kp + off < npw) {
+        if (i < 21) {
+          c = (int)((cnts >> (3 * i)) & 7ull);
+        } else {
+          ContactSink s{&d, 0, 0, 0, w, false, 0, 0};
+          narrowphase_pair(m, gxpos, gxmat, g1, g2, s);
+          c = s.k;
+        }
+      }
+    }
+    kp += tot;
+    int ctot;
+    const int coff = team_exscan_bits<TEAM, 3>(c, &ctot);
+    if (c > 0) {
+      ContactSink s{&d, cbase, cap, kc + coff, w, true, 0, 0};
+      narrowphase_pair(m, gxpos, gxmat, g1, g2, s);
+      const int n = min(cbase + kc + coff + c, cap) - (cbase + kc + coff);
+      if (n > 0) contact_params(m, d, g1, g2, cbase + kc + coff, n);
+    }
+    kc += ctot;
+  }
+  sync();
 }
 
 }  // namespace mjh

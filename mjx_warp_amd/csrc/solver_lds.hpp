@@ -29,6 +29,42 @@ __host__ __device__ inline int lds_solver_floats(int nv, int rmax, int rj) {
   return nv * nv + 14 * nv + 8 * rmax + rj * nv;
 }
 
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// Lower triangle of H = M + J' diag(dr) J (solver.py:250-276 _JTDAJ plus the
+// mass matrix) on the matrix cores, for one world per 64-lane wavefront:
+// H is cut into 32 x 32 tiles; tile (ti, tj), tj <= ti, accumulates
+// v_mfma_f32_32x32x2f32 over the rows two at a time.  Lane l supplies row
+// k + l / 32 of the world at column l % 32 of the tile: dr J[r][ti*32 + c]
+// as the A operand, J[r][tj*32 + c] as the B operand; result entry v of lane
+// l is tile row 8 (v / 4) + 4 (l / 32) + v % 4, tile column l % 32.  Every
+// lane of the wavefront must call it (the matrix instruction uses all 64).
+MJH_DEV void hessian_mfma(const float* M, const float* J, const float* D, const float* act, int nr, int nv,
+                          float* H, int lane) {
+  const int c = lane & 31, half = lane >> 5;
+  const int nt = (nv + 31) >> 5;
+  for (int ti = 0; ti < nt; ti++)
+    for (int tj = 0; tj <= ti; tj++) {
+      floatx16 acc = {};
+      const int ci = ti * 32 + c, cj = tj * 32 + c;
+      for (int k = 0; k < nr; k += 2) {
+        const int r = k + half;
+        float a = 0.0f, b = 0.0f;
+        if (r < nr) {
+          const float dr = D[r] * act[r];
+          if (ci < nv) a = dr * J[r * nv + ci];
+          if (cj < nv) b = J[r * nv + cj];
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const int i = ti * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
+        if (i < nv && cj < nv && cj <= i) H[i * nv + cj] = M[i * nv + cj] + acc[v];
+      }
+    }
+}
+
 template <int TEAM>
 struct LdsSolver {
   const mjhModel& m;
@@ -90,7 +126,12 @@ struct LdsSolver {
       gd += g * g;
     }
     gd = team_sum<TEAM>(gd);
-    if (m.opt.solver == mjhSOL_NEWTON) {
+    if (m.opt.solver == mjhSOL_NEWTON && TEAM == 64) {
+      hessian_mfma(M, J, D, act, nr, nv, H, t.lane);
+      sync();
+      chol_lds(H, nv, nv, m, t);
+      cho_solve_lds(H, nv, nv, x, t);
+    } else if (m.opt.solver == mjhSOL_NEWTON) {
       const int ntri = nv * (nv + 1) / 2;
       for (int p = t.lane; p < ntri; p += TEAM) {
         const int i = m.dof_tri_row[p], j = m.dof_tri_col[p];

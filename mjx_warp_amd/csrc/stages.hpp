@@ -123,6 +123,21 @@ struct StTail {
   }
 };
 
+// Collision and constraint rows inside a fused launch: for nxn models the
+// three pool reservations are in flight together (collide_and_reserve),
+// SAP models take the stage functions one after the other.
+template <int T>
+MJH_DEV void fused_collision_rows(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t) {
+  if (m.ngeom <= 100) {
+    int rbase, rtotal, nlim;
+    collide_and_reserve(m, d, l, t, &rbase, &rtotal, &nlim);
+    if (!(m.opt.disableflags & mjhDSBL_CONSTRAINT)) make_constraint_rows(m, d, l, t, rbase, rtotal, nlim);
+  } else {
+    collision(m, d, l, t);
+    make_constraint(m, d, l, t);
+  }
+}
+
 // The whole step of one world (forward.py:682-695 / 664-679).
 struct StFusedStep {
   template <int T>
@@ -132,8 +147,7 @@ struct StFusedStep {
     crb(m, d, l, t);
     factor_m(m, d, l, t);
     sync();
-    collision(m, d, l, t);
-    make_constraint(m, d, l, t);
+    fused_collision_rows(m, d, l, t);
     sync();
     transmission(m, d, t);
     fwd_velocity(m, d, l, t);
@@ -166,8 +180,7 @@ struct StFusedForward {
     com_pos(m, d, l, t);
     crb(m, d, l, t);
     sync();
-    collision(m, d, l, t);
-    make_constraint(m, d, l, t);
+    fused_collision_rows(m, d, l, t);
     sync();
     transmission(m, d, t);
     fwd_velocity(m, d, l, t);

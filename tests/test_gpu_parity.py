@@ -212,11 +212,32 @@ def test_forward_humanoid_truncated_solver(solver, iterations, ls_iterations, ls
   mjw.forward(m, d)
   torch.cuda.synchronize()
   oracle.run("forward", hm, hd)
-  env = oracle_envelope(mjm, 64, 1, 64, 277, "forward", ["efc_cost"], key=0,
-                        opt={"ls_parallel": int(ls_parallel)})["efc_cost"]
+  env = oracle_envelope(mjm, 64, 1, 64, 277, "forward", ["efc_cost", "qacc", "qfrc_constraint"], key=0,
+                        opt={"ls_parallel": int(ls_parallel)})
   for f in SMOOTH_FIELDS:
     _close(f, helpers.data_numpy(d, f), hd[f], 2e-4)
-  _cost_parity(d, hd["efc_cost"], env)
+  _cost_parity(d, hd["efc_cost"], env["efc_cost"])
+  # field-wise, for Newton (the reference's own solver test holds Newton
+  # solves field by field and CG solves by cost only, solver_test.py:
+  # 127-130): worlds where the oracle's float32, float64 and perturbed runs
+  # agree are held to the plain tolerance, the others to twice the oracle's
+  # own spread.  Truncated CG walks flat cost valleys whose end point moves
+  # with the summation order more than with 1e-6 input perturbations; its
+  # field errors are printed, its parity is the cost check above.
+  for f in ("qacc", "qfrc_constraint"):
+    got = helpers.data_numpy(d, f).astype(np.float64)
+    ref = hd[f].astype(np.float64)
+    spread = np.abs(env[f] - ref[None]).max(axis=(0, 2))
+    err = np.abs(got - ref).max(axis=1)
+    scale = 1.0 + np.abs(ref).max()
+    agree = spread <= 2e-3 * scale
+    print(f"truncated {solver}/{iterations}/{ls_iterations}/{int(ls_parallel)} {f}: "
+          f"{agree.sum()} of {len(agree)} worlds rounding-stable, max err there {err[agree].max():.2e}, "
+          f"elsewhere max err/spread {np.max(err[~agree] / spread[~agree]) if (~agree).any() else 0:.2f}", flush=True)
+    if solver != 2:
+      continue
+    assert np.all(err[agree] <= 2e-3 * scale), (f, np.where(agree & (err > 2e-3 * scale))[0])
+    assert np.all(err[~agree] <= 2.0 * spread[~agree] + 2e-3 * scale), f
 
 
 def test_forward_chain():
@@ -344,9 +365,13 @@ def _perturbed(mjm, nworld, seed, scale=0.1, key=None):
   return q, rng.normal(0, 0.5, (nworld, mjm.nv))
 
 
+@pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("name,integrator", [("pendula.xml", 0), ("pendula.xml", 3), ("constraints.xml", 0),
                                              ("constraints.xml", 3), ("collision.xml", 0)])
-def test_forward_and_step_reference_scenes(name, integrator):
+def test_forward_and_step_reference_scenes(name, integrator, fused):
+  """The reference's test scenes: forward field by field, then one plain or
+  fused step (pendula, nv 33, and constraints, nv 50, take the fused step's
+  64-lane one-launch form)."""
   from mjx_warp_amd import mjcf
 
   mjm = mjcf.from_xml_path(str(helpers_golden(name)))
@@ -356,11 +381,12 @@ def test_forward_and_step_reference_scenes(name, integrator):
   q, v = _perturbed(mjm, nworld, seed=5, key=1 if mjm.nkey > 1 else (0 if mjm.nkey else None))
   u = np.random.default_rng(6).uniform(-1, 1, (nworld, mjm.nu))
   m, d, hm, hd = _gpu_and_oracle(mjm, nworld, q, v, u)
-  mjw.forward(m, d)
-  torch.cuda.synchronize()
-  oracle.run("forward", hm, hd)
-  compare_forward(m, d, hd)
-  mjw.step(m, d)
+  if not fused:
+    mjw.forward(m, d)
+    torch.cuda.synchronize()
+    oracle.run("forward", hm, hd)
+    compare_forward(m, d, hd)
+  (mjw.step_fused if fused else mjw.step)(m, d)
   torch.cuda.synchronize()
   oracle.run("step", hm, hd)
   _close("qpos", helpers.data_numpy(d, "qpos"), hd["qpos"], 2e-4)
@@ -509,3 +535,4 @@ def test_support_functions():
     oracle.run(s, hm, hd)
   oracle.run("fwd_acceleration", hm, hd)
   _close("xfrc", q.cpu().numpy(), hd["qfrc_smooth"], 2e-4)
+

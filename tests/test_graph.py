@@ -109,3 +109,33 @@ def test_fused_step_description_path_equals_argument_path(fused):
       a = helpers.data_numpy(d_desc, f)
       b = helpers.data_numpy(d_arg, f)
       assert np.array_equal(a, b), (lying, f, np.abs(a - b).max())
+
+
+def test_fused_step_description_path_equals_argument_path_64_lane():
+  """The same for a model over 32 dofs (a 40-link chain hung so that its
+  lower links rest on the floor): the one-launch fused step of 64-lane
+  teams, from Data.step_desc and from its kernel arguments."""
+  from mjx_warp_amd import mjcf
+
+  mjm = mjcf.from_xml_string(models.chain_xml(40, anchor_z=19.0))
+  mjm.opt.iterations = 20
+  m = mjw.put_model(mjm)
+  rng = np.random.default_rng(8)
+  q = rng.uniform(-0.4, 0.4, (16, mjm.nq))
+  v = rng.normal(0, 0.3, (16, mjm.nv))
+  out = []
+  for desc in (True, False):
+    d = mjw.make_data(mjm, nworld=16, nconmax=64 * 16, njmax=256 * 16)
+    helpers.set_field(d, "qpos", q)
+    helpers.set_field(d, "qvel", v)
+    if not desc:
+      d.ndesc = 0
+      d.step_desc = torch.zeros(0, dtype=torch.int32, device="cuda")
+    for _ in range(3):
+      mjw.step_fused(m, d)
+    out.append(d)
+  torch.cuda.synchronize()
+  assert int(helpers.data_numpy(out[0], "nefc")[0]) > 0
+  for f in ("qpos", "qvel", "qacc", "qacc_warmstart"):
+    a, b = helpers.data_numpy(out[0], f), helpers.data_numpy(out[1], f)
+    assert np.isfinite(a).all() and np.array_equal(a, b), (f, np.abs(a - b).max())

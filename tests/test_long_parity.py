@@ -11,8 +11,11 @@ GPU, plain and fused.
   own state (qpos, qvel, qacc_warmstart, ctrl) and its qpos, qvel, qacc and
   per-world efc_force are compared with the GPU's next state -- as the
   single-step tests do, with the float32 oracle's rounding envelope
-  (float64 and perturbed float32 runs) for worlds whose truncated solve is
-  rounding-sensitive.
+  (float64 and eight perturbed float32 runs) for worlds whose solve is
+  rounding-sensitive; a world outside both is held to the reference's
+  solver criterion, its cost within 2.5% of the oracle runs' (CG in a flat
+  cost valley ends at points that move with the summation order,
+  solver_test.py:127).  The count of such worlds is printed.
 * Free-running envelope at step 1000: the float32 oracle runs all 1000 steps
   from the same initial state and controls, beside the float64 oracle and
   two float32 runs whose hinge coordinates start 1e-6 apart.  Contact-rich
@@ -35,7 +38,26 @@ from mjx_warp_amd import models
 from oracle import oracle
 
 import helpers
-from test_gpu_parity import _close_or_enveloped
+
+
+def _resync_close(name, got, ref, env, tol, cost, cost_env, factor=1.025):
+  """Per world: within tol * (1 + max|ref|), or within twice the oracle's
+  spread env [nvariant, nworld, ...], or with the world's solver cost
+  inside the oracle runs' costs widened by the reference's factor.
+  Returns the worlds that needed the cost criterion."""
+  got = np.asarray(got, np.float64).reshape(ref.shape[0], -1)
+  ref = np.asarray(ref, np.float64).reshape(ref.shape[0], -1)
+  env = np.asarray(env, np.float64).reshape(env.shape[0], ref.shape[0], -1)
+  err = np.abs(got - ref).max(axis=1)
+  spread = np.abs(env - ref[None]).max(axis=(0, 2))
+  ok = (err <= tol * (1.0 + np.abs(ref).max())) | (err <= 2.0 * spread)
+  lo = cost_env.min(axis=0) / factor - 1e-6
+  hi = cost_env.max(axis=0) * factor + 1e-6
+  by_cost = ~ok & (cost >= lo) & (cost <= hi)
+  bad = ~ok & ~by_cost
+  assert not np.any(bad), f"{name}: worlds {np.where(bad)[0].tolist()} err {err[bad]} spread {spread[bad]}"
+  return np.where(by_cost)[0].tolist()
+
 
 pytestmark = pytest.mark.gpu
 
@@ -81,7 +103,7 @@ class OracleRun:
     h = self.hd
     return dict(qpos=h["qpos"].astype(np.float64).copy(), qvel=h["qvel"].astype(np.float64).copy(),
                 qacc=h["qacc"].astype(np.float64).copy(), world_efc=h["world_efc"].copy(),
-                efc_force=h["efc_force"].astype(np.float64).copy())
+                efc_force=h["efc_force"].astype(np.float64).copy(), cost=h["efc_cost"].astype(np.float64).copy())
 
 
 def _efc_by_world(f, nworld):
@@ -93,7 +115,8 @@ def _gpu_fields(d):
   return dict(qpos=helpers.data_numpy(d, "qpos").astype(np.float64), qvel=helpers.data_numpy(d, "qvel").astype(np.float64),
               qacc=helpers.data_numpy(d, "qacc").astype(np.float64), world_efc=helpers.data_numpy(d, "world_efc"),
               efc_force=helpers.data_numpy(d, "efc_force").astype(np.float64),
-              warm=helpers.data_numpy(d, "qacc_warmstart").astype(np.float64))
+              warm=helpers.data_numpy(d, "qacc_warmstart").astype(np.float64),
+              cost=helpers.data_numpy(d, "efc_cost").astype(np.float64))
 
 
 @pytest.mark.parametrize("fused", [False, True])
@@ -113,6 +136,7 @@ def test_thousand_steps(name, fused):
   worst32 = {k: 0.0 for k in ("qpos", "qvel", "qacc", "efc_force")}
   worst64 = dict(worst32)
   rows_differ = 0
+  by_cost = set()
   for t in range(NSTEP):
     snap = t % EVERY == 0
     if snap:
@@ -133,8 +157,8 @@ def test_thousand_steps(name, fused):
       ref[key] = orun.fields()
     # the float32 oracle's rounding envelope at this state
     rng = np.random.default_rng(t)
-    env = {k: [ref["f64"][k]] for k in ("qpos", "qvel", "qacc")}
-    for _ in range(4):
+    env = {k: [ref["f64"][k]] for k in ("qpos", "qvel", "qacc", "cost")}
+    for _ in range(8):
       qp = before["qpos"].copy()
       qp[:, hinge] += 1e-6 * rng.normal(size=(nworld, len(hinge)))
       o32.set(qp, before["qvel"] * (1 + 1e-6 * rng.normal(size=before["qvel"].shape)), before["warm"])
@@ -142,8 +166,10 @@ def test_thousand_steps(name, fused):
       f = o32.fields()
       for k in env:
         env[k].append(f[k])
+    cost_env = np.stack([ref["f32"]["cost"]] + env["cost"])
     for k, tol in (("qpos", 1e-4), ("qvel", 2e-3), ("qacc", 2e-3)):
-      _close_or_enveloped(f"{k} at step {t}", after[k], ref["f32"][k], np.stack(env[k]), tol)
+      by_cost.update(_resync_close(f"{k} at step {t}", after[k], ref["f32"][k], np.stack(env[k]), tol,
+                                   after["cost"], cost_env))
       worst32[k] = max(worst32[k], float(np.abs(after[k] - ref["f32"][k]).max()))
       worst64[k] = max(worst64[k], float(np.abs(after[k] - ref["f64"][k]).max()))
     g = _efc_by_world(after, nworld)
@@ -187,4 +213,5 @@ def test_thousand_steps(name, fused):
                    oracle_spread_median=float(np.median(spread)), worlds_inside=float(np.mean(inside)))
   print("LONG_PARITY " + json.dumps(dict(workload=name, fused=fused, steps=NSTEP, every=EVERY,
                                          resync_max_err_vs_f32=worst32, resync_max_err_vs_f64=worst64,
-                                         rows_differ=rows_differ, free_running_at_1000=free)), flush=True)
+                                         rows_differ=rows_differ, worlds_held_by_cost=len(by_cost),
+                                         free_running_at_1000=free)), flush=True)
