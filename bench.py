@@ -60,15 +60,16 @@ PROFILE = os.path.join(ROOT, "profiles", "step_humanoid_b8192_newton10.md")
 
 
 def kernel_sources_sha() -> str:
-  """sha256 over the HIP sources (csrc/*.hpp, *.hip in name order): the
-  profile summary records it, so a measured figure is used only for the
-  kernels it was measured on."""
+  """sha256 over the stage kernels' sources (csrc/*.hpp and kernels.hip in
+  name order; api.hip holds the host code): the profile summary records
+  it, so a measured figure is used only for the kernels it was measured
+  on."""
   import glob
   import hashlib
 
   h = hashlib.sha256()
   for path in sorted(glob.glob(os.path.join(ROOT, "mjx_warp_amd", "csrc", "*.hpp"))
-                     + glob.glob(os.path.join(ROOT, "mjx_warp_amd", "csrc", "*.hip"))):
+                     + [os.path.join(ROOT, "mjx_warp_amd", "csrc", "kernels.hip")]):
     with open(path, "rb") as f:
       h.update(os.path.basename(path).encode() + b"\0" + f.read())
   return h.hexdigest()[:16]

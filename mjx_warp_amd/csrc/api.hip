@@ -348,6 +348,36 @@ int check_hip(hipError_t e, const char* what) {
   return MJH_OK;
 }
 
+// LDS bytes one workgroup may take on the current device (160 KiB on
+// gfx950), read once per device.
+size_t lds_cap() {
+  static std::mutex mu;
+  static std::map<int, size_t> per_device;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 65536;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = per_device.find(dev);
+  if (it != per_device.end()) return it->second;
+  int v = 0;
+  size_t cap = 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0) cap = v;
+  else (void)hipGetLastError();
+  per_device[dev] = cap;
+  return cap;
+}
+// A launch of `kernel` with `bytes` of dynamic LDS: above 64 KiB the kernel
+// is allowed that much first.
+int allow_lds(const void* kernel, size_t bytes, const char* name) {
+  if (bytes <= 65536) return MJH_OK;
+  if (bytes > lds_cap())
+    return fail(MJH_ERR_CAPACITY, std::string(name) + ": needs " + std::to_string(bytes) + " bytes of LDS per workgroup, the device has " +
+                                      std::to_string(lds_cap()));
+  return check_hip(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes), name);
+}
+
 // Runs side(stream) beside main(s): side on the side stream, forked after
 // the work already queued on s and joined before what s runs next; in order
 // on s when there is no side stream or use_side is false.
@@ -374,9 +404,10 @@ int launch_team(const mjhModel& m, const mjhData& d, hipStream_t s, int ldsw, co
   if (wpb > 1 && ldsw % 4 != 0) return fail(MJH_ERR_INVALID, std::string(name) + ": LDS stride not 16-byte aligned");
   const int grid = (d.nworld + wpb - 1) / wpb;
   const size_t bytes = (size_t)ldsw * wpb * sizeof(float);
-  if (bytes > 65536)
+  if (bytes > lds_cap())
     return fail(MJH_ERR_CAPACITY, std::string(name) + ": model too large for the dense LDS kernels (nv = " +
                                       std::to_string(m.nv) + ")");
+  TRY(allow_lds((const void*)k_stage<TEAM, S>, bytes, name));
   hipLaunchKernelGGL((k_stage<TEAM, S>), dim3(grid), dim3(64), bytes, s, m, d, ldsw, tune ? *tune : tune_for(m));
   return check_launch(name);
 }
@@ -541,6 +572,7 @@ int launch_groups(const GroupTable& t, int ldsw, hipStream_t s, const char* name
   constexpr int wpb = 64 / TEAM;
   if (wpb > 1 && ldsw % 4 != 0) return fail(MJH_ERR_INVALID, std::string(name) + ": LDS stride not 16-byte aligned");
   const size_t bytes = (size_t)ldsw * wpb * sizeof(float);
+  TRY(allow_lds((const void*)k_stage_groups<TEAM, S, TI>, bytes, name));
   hipLaunchKernelGGL((k_stage_groups<TEAM, S, TI>), dim3(t.blocks), dim3(64), bytes, s, t.dev, t.ngroups, ldsw);
   return check_launch(name);
 }
@@ -577,7 +609,7 @@ GroupDesc single_desc(const mjhModel& m, const mjhData& d) {
 // group kernels, as the fused step does; counters are left alone.
 int solve_reg_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
   const int l_solve = lds_solve_reg(m), l_big = lds_tail_reg(m);
-  if ((size_t)max_of(2 * l_solve, 2 * l_big) * sizeof(float) > 65536)
+  if ((size_t)max_of(2 * l_solve, 2 * l_big) * sizeof(float) > lds_cap())
     return fail(MJH_ERR_CAPACITY, "solve: model too large for the dense LDS kernels (nv = " + std::to_string(m.nv) + ")");
   GroupDesc* dev = reinterpret_cast<GroupDesc*>(d.step_desc);
   hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(64), 0, s, single_desc(m, d), dev, 0);
@@ -604,7 +636,7 @@ int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
             l_tail = lds_linalg(m);
   const size_t need = reg ? (size_t)max_of(max_of(2 * l_fwd, 2 * l_solve), max_of(2 * l_big, l_tail))
                           : (size_t)max_of(l_step, l_tail);
-  if (need * sizeof(float) > 65536)
+  if (need * sizeof(float) > lds_cap())
     return fail(MJH_ERR_CAPACITY,
                 "step_fused: model too large for the dense LDS kernels (nv = " + std::to_string(m.nv) + ")");
   hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(64), 0, s, g, dev, (m.opt.disableflags & mjhDSBL_CONSTRAINT) ? 1 : 3);
@@ -658,7 +690,7 @@ int plan_group(int i, const mjhModel& m, const mjhData& d, mjhMulti* plan, Group
 // each launch's LDS: two worlds per workgroup on 32-lane teams, one on 64
 int plan_capacity(const mjhMulti& plan) {
   const int need = max_of(max_of(2 * plan.lds_fwd, 2 * plan.lds_solve), max_of(2 * plan.lds_big, plan.lds_tail32));
-  if ((size_t)max_of(need, max_of(plan.lds_step, plan.lds_tail64)) * sizeof(float) > 65536)
+  if ((size_t)max_of(need, max_of(plan.lds_step, plan.lds_tail64)) * sizeof(float) > lds_cap())
     return fail(MJH_ERR_CAPACITY, "multi: a group's model is too large for the dense LDS kernels");
   return MJH_OK;
 }
@@ -828,7 +860,8 @@ int mjh_rollout(const mjhModel* m, const mjhData* d, int nstep, int fused, mjhSt
 int mjh_solve_m(const mjhModel* m, const mjhData* d, float* x, const float* y, mjhStream stream) {
   if (d->nworld <= 0) return MJH_OK;
   const int ldsw = lds_linalg(*m);
-  if ((size_t)ldsw * sizeof(float) > 65536) return fail(MJH_ERR_CAPACITY, "solve_m: nv too large");
+  if ((size_t)ldsw * sizeof(float) > lds_cap()) return fail(MJH_ERR_CAPACITY, "solve_m: nv too large");
+  TRY(allow_lds((const void*)k_solve_m, (size_t)ldsw * sizeof(float), "solve_m"));
   hipLaunchKernelGGL(k_solve_m, dim3(d->nworld), dim3(64), ldsw * sizeof(float), (hipStream_t)stream, *m, *d, x, y,
                      ldsw);
   return check_launch("solve_m");

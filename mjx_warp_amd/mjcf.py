@@ -295,8 +295,19 @@ def _resolve(defaults, kind, el, childclass):
 
 
 class _Compiler:
-  def __init__(self, root: ET.Element):
+  def __init__(self, root: ET.Element, extra_classes=None):
     self.root = root
+    self._settings(root)
+    self.defaults = _Defaults()
+    dflt = root.find("default")
+    if dflt is not None:
+      self.defaults.build(dflt)
+    else:
+      self.defaults.build(ET.Element("default"))
+    # default classes of attached models, under their prefixed names
+    self.defaults.classes.update(extra_classes or {})
+
+  def _settings(self, root):
     comp = root.find("compiler")
     self.angle_deg = True
     self.autolimits = True
@@ -305,12 +316,6 @@ class _Compiler:
       self.angle_deg = comp.get("angle", "degree") == "degree"
       self.autolimits = comp.get("autolimits", "true") == "true"
       self.eulerseq = comp.get("eulerseq", "xyz")
-    self.defaults = _Defaults()
-    dflt = root.find("default")
-    if dflt is not None:
-      self.defaults.build(dflt)
-    else:
-      self.defaults.build(ET.Element("default"))
 
   # ---- orientation ----
   def _angle(self, a):
@@ -1032,6 +1037,183 @@ def set_const(m: MjModel):
 
 
 # ---------------------------------------------------------------------------
+# <replicate>, <frame> and <attach> (with <model file=...> assets): expanded
+# into a plain worldbody before compiling.
+#
+# * frame: its position / orientation compose into its children's (bodies,
+#   geoms, sites; a geom's fromto endpoints move with it);
+# * replicate count=N: N copies of its children, copy k under the frame of
+#   k times `offset` and k compositions of its rotation (euler / quat /
+#   axisangle), every name in copy k suffixed with sep + k;
+# * attach model=M body=B prefix=P: a copy of body B of the asset model M
+#   (an MJCF file beside the including one) with every name prefixed by P,
+#   the orientations of M's subtree converted with M's own compiler
+#   settings, M's default classes merged under prefixed names (M's "main"
+#   becomes P + "main", the attached body's default class), and M's
+#   actuators of the subtree's joints appended (names prefixed, and suffixed
+#   inside a replicate).
+# Lights and cameras, which the step does not use, are dropped in expanded
+# subtrees.
+
+_NAMED = ("body", "joint", "freejoint", "geom", "site")
+
+
+class _Expander:
+  def __init__(self, root, base_dir):
+    self.root = root
+    self.base_dir = base_dir
+    self.comp = _Compiler.__new__(_Compiler)  # orientation() of the including model
+    _Compiler._settings(self.comp, root)
+    self.models = {}
+    self.classes = {}
+    self.actuators = []
+
+  def _model(self, name):
+    if name in self.models:
+      return self.models[name]
+    asset = self.root.find("asset")
+    el = None if asset is None else next((a for a in asset.findall("model") if a.get("name") == name), None)
+    if el is None or "file" not in el.attrib:
+      raise ValueError(f"attach: no asset <model name={name!r} file=...>")
+    if self.base_dir is None:
+      raise NotImplementedError("attach of a model file needs the including file's path (from_xml_path)")
+    import os
+
+    path = os.path.join(self.base_dir, el.get("file"))
+    with open(path) as f:
+      sub = ET.fromstring(f.read())
+    sub_base = os.path.dirname(os.path.abspath(path))
+    sub, sub_classes, sub_acts = _Expander(sub, sub_base).run()
+    comp = _Compiler.__new__(_Compiler)
+    _Compiler._settings(comp, sub)
+    if comp.angle_deg != self.comp.angle_deg:
+      raise NotImplementedError("attach: the attached model's angle unit differs from the including model's")
+    dflt = _Defaults()
+    dflt.build(sub.find("default") if sub.find("default") is not None else ET.Element("default"))
+    dflt.classes.update(sub_classes)
+    self.models[name] = (sub, comp, dflt, sub_acts)
+    return self.models[name]
+
+  def run(self):
+    world = self.root.find("worldbody")
+    if world is not None:
+      self._expand(world)
+    act = self.root.find("actuator")
+    if self.actuators:
+      if act is None:
+        act = ET.SubElement(self.root, "actuator")
+      for a in self.actuators:
+        act.append(a)
+    return self.root, self.classes, self.actuators
+
+  # in place: frames, replicates and attaches among el's children (and
+  # deeper, inside bodies) become plain elements
+  def _expand(self, el):
+    out = []
+    for child in list(el):
+      out.extend(self._flatten(child, np.zeros(3), np.array([1.0, 0, 0, 0]), ""))
+    for child in list(el):
+      el.remove(child)
+    for c in out:
+      el.append(c)
+
+  def _flatten(self, el, pos, quat, suffix):
+    """el as plain elements under the frame (pos, quat), names suffixed."""
+    tag = el.tag
+    if tag == "frame":
+      fp, fq = self._frame_of(el, self.comp)
+      p2, q2 = pos + _qrot(quat, fp), _qmul(quat, fq)
+      return [c for child in el for c in self._flatten(child, p2, q2, suffix)]
+    if tag == "replicate":
+      count = int(el.get("count", "1"))
+      sep = el.get("sep", "")
+      off = np.array(_floats(el.get("offset", "0 0 0")))
+      _, rq = self._frame_of(el, self.comp)
+      width = len(str(max(count - 1, 0)))
+      out, qk = [], np.array([1.0, 0, 0, 0])
+      for k in range(count):
+        p2, q2 = pos + _qrot(quat, k * off), _qmul(quat, qk)
+        for child in el:
+          out.extend(self._flatten(child, p2, q2, suffix + sep + str(k).zfill(width)))
+        qk = _qmul(qk, rq)
+      return out
+    if tag == "attach":
+      return [self._attach(el, pos, quat, suffix)]
+    if tag in ("light", "camera") and (suffix or np.any(pos) or quat[0] != 1.0):
+      return []
+    c = copy.deepcopy(el)
+    if tag in ("body", "geom", "site"):
+      self._place(c, pos, quat, self.comp)
+    _rename(c, "", suffix)
+    if tag == "body":
+      self._expand(c)
+    return [c]
+
+  def _frame_of(self, el, comp):
+    a = _attrs(el)
+    return np.array(a.get("pos", [0.0, 0.0, 0.0]), dtype=np.float64), comp.orientation(a)
+
+  @staticmethod
+  def _place(c, pos, quat, comp):
+    """c's own frame moved by (pos, quat); orientation written as quat."""
+    a = _attrs(c)
+    if "fromto" in a:
+      f = np.array(a["fromto"], dtype=np.float64)
+      f0, f1 = pos + _qrot(quat, f[:3]), pos + _qrot(quat, f[3:])
+      c.set("fromto", " ".join(repr(float(x)) for x in np.concatenate([f0, f1])))
+      return
+    p = np.array(a.get("pos", [0.0, 0.0, 0.0]), dtype=np.float64)
+    q = comp.orientation(a)
+    for k in ("quat", "euler", "axisangle", "zaxis", "xyaxes"):
+      c.attrib.pop(k, None)
+    p2, q2 = pos + _qrot(quat, p), _qmul(quat, q)
+    c.set("pos", " ".join(repr(float(x)) for x in p2))
+    c.set("quat", " ".join(repr(float(x)) for x in q2))
+
+  def _attach(self, el, pos, quat, suffix):
+    sub, comp, dflt, sub_acts = self._model(el.get("model"))
+    prefix = el.get("prefix", "")
+    body = next((b for b in sub.iter("body") if b.get("name") == el.get("body")), None)
+    if body is None:
+      raise ValueError(f"attach: model {el.get('model')!r} has no body {el.get('body')!r}")
+    c = copy.deepcopy(body)
+    # the subtree's orientations in the attached model's own convention
+    for e in c.iter():
+      if e is not c and e.tag in ("body", "geom", "site"):
+        self._place(e, np.zeros(3), np.array([1.0, 0, 0, 0]), comp)
+    self._place(c, np.zeros(3), np.array([1.0, 0, 0, 0]), comp)
+    self._place(c, pos, quat, self.comp)
+    for e in c.iter():
+      for k in ("class", "childclass"):
+        if k in e.attrib:
+          e.set(k, prefix + e.get(k))
+    if "childclass" not in c.attrib:
+      c.set("childclass", prefix + "main")
+    for name, cls in dflt.classes.items():
+      self.classes[prefix + name] = copy.deepcopy(cls)
+    joints = {e.get("name") for e in c.iter() if e.tag in ("joint", "freejoint") and e.get("name")}
+    _rename(c, prefix, suffix)
+    act = sub.find("actuator")
+    for a in [] if act is None else list(act):
+      if a.get("joint") in joints:
+        a2 = copy.deepcopy(a)
+        a2.set("joint", prefix + a.get("joint") + suffix)
+        if a2.get("name"):
+          a2.set("name", prefix + a2.get("name") + suffix)
+        a2.set("class", prefix + a.get("class", "main"))
+        self.actuators.append(a2)
+    return c
+
+
+def _rename(el, prefix, suffix):
+  if not prefix and not suffix:
+    return
+  for e in el.iter():
+    if e.tag in _NAMED and e.get("name"):
+      e.set("name", prefix + e.get("name") + suffix)
+
+
+# ---------------------------------------------------------------------------
 # entry points
 
 _FIX_ATTR = re.compile(r'(=\s*"[^"]*")(?=[A-Za-z_])')
@@ -1039,11 +1221,12 @@ _FIX_ATTR = re.compile(r'(=\s*"[^"]*")(?=[A-Za-z_])')
 
 # Elements that change the model's structure and that this compiler does
 # not expand: compiling without them would silently drop bodies.
-_UNSUPPORTED_ELEMENTS = ("replicate", "frame", "attach", "composite", "flexcomp", "include", "plugin")
+_UNSUPPORTED_ELEMENTS = ("composite", "flexcomp", "include", "plugin")
 
 
-def from_xml_string(xml: str) -> MjModel:
-  """Compiles an MJCF string (MjModel.from_xml_string counterpart)."""
+def from_xml_string(xml: str, base_dir=None) -> MjModel:
+  """Compiles an MJCF string (MjModel.from_xml_string counterpart); model
+  files named by <attach> are read relative to base_dir."""
   try:
     root = ET.fromstring(xml)
   except ET.ParseError:
@@ -1051,13 +1234,14 @@ def from_xml_string(xml: str) -> MjModel:
     # (e.g. type="ball"solreflimit=...); insert it and retry.
     root = ET.fromstring(_FIX_ATTR.sub(r"\1 ", xml))
   for el in root.iter():
-    if el.tag in _UNSUPPORTED_ELEMENTS or (el.tag == "model" and "file" in el.attrib):
+    if el.tag in _UNSUPPORTED_ELEMENTS:
       raise NotImplementedError(f"MJCF element <{el.tag}> is not supported by this compiler")
+  root, classes, _ = _Expander(root, base_dir).run()
   if root.find("equality") is not None and len(root.find("equality")):
-    m = _Compiler(root).compile()
+    m = _Compiler(root, classes).compile()
     m.neq = len(root.find("equality"))
     return m
-  m = _Compiler(root).compile()
+  m = _Compiler(root, classes).compile()
   if root.find("sensor") is not None:
     m.nsensor = len(root.find("sensor"))
   if root.find("tendon") is not None:
@@ -1066,5 +1250,7 @@ def from_xml_string(xml: str) -> MjModel:
 
 
 def from_xml_path(path: str) -> MjModel:
+  import os
+
   with open(path) as f:
-    return from_xml_string(f.read())
+    return from_xml_string(f.read(), base_dir=os.path.dirname(os.path.abspath(path)))

@@ -67,8 +67,11 @@ def benchmark(
   njmax: int = -1,
   event_trace: bool = False,
   measure_alloc: bool = False,
+  graph: bool = True,
 ) -> Tuple[float, float, dict, int, list, list]:
-  """Times nstep calls of fn on batch_size copies of mjd (test_util.py:88-120)."""
+  """Times nstep calls of fn on batch_size copies of mjd (test_util.py:88-120).
+  graph=False (no counterpart in the reference) calls fn eagerly instead of
+  replaying a captured graph, e.g. for counter collection by a profiler."""
   import torch
 
   if solver == "cg":
@@ -98,6 +101,16 @@ def benchmark(
         fn(m, d)
         step_trace = tracer.trace()
       trace = _sum(trace, step_trace) if trace else step_trace
+      if measure_alloc:
+        ncon.append(int(d.ncon[0].item()))
+        nefc.append(int(d.nefc[0].item()))
+    torch.cuda.synchronize()
+    run_duration = time.perf_counter() - run_beg
+  elif not graph:
+    torch.cuda.synchronize()
+    run_beg = time.perf_counter()
+    for _ in range(nstep):
+      fn(m, d)
       if measure_alloc:
         ncon.append(int(d.ncon[0].item()))
         nefc.append(int(d.nefc[0].item()))

@@ -68,6 +68,8 @@ def _parser() -> argparse.ArgumentParser:
   p.add_argument("--event_trace", type=_bool, nargs="?", const=True, default=False, help="Provide a full event trace")
   p.add_argument("--measure_alloc", type=_bool, nargs="?", const=True, default=False,
                  help="Measure how much of nconmax, njmax is used.")
+  p.add_argument("--graph", type=_bool, nargs="?", const=True, default=True,
+                 help="replay a captured HIP graph (false: call the function eagerly, e.g. under a profiler)")
   return p
 
 
@@ -113,7 +115,8 @@ def main(argv: Sequence[str] | None = None) -> int:
   print(f"Rolling out {args.nstep} steps at dt = {m.opt.timestep:.3f}...")
   jit_time, run_time, trace, steps, ncon, nefc = benchmark(
     getattr(mjx_warp_amd, args.function), m, d, args.nstep, args.batch_size, args.solver, args.iterations,
-    args.ls_iterations, args.ls_parallel, args.nconmax, args.njmax, args.event_trace, args.measure_alloc)
+    args.ls_iterations, args.ls_parallel, args.nconmax, args.njmax, args.event_trace, args.measure_alloc,
+    graph=args.graph)
 
   if args.output == "text":
     print(f"""
