@@ -81,7 +81,7 @@ def make_case(mjm, nworld, seed, nconmax_w, njmax_w, key=None, qpos_noise=0.05, 
 
 
 def oracle_envelope(mjm, nworld, seed, nconmax_w, njmax_w, stage, fields, key=None, qpos_noise=0.05,
-                    qvel_noise=0.5, ctrl=True, opt=None, k=8):
+                    qvel_noise=0.5, ctrl=True, opt=None, k=8, inputs=None):
   """The oracle's own rounding envelope of one seeded batch: `stage` run in
   float64 and, k times, in float32 with the hinge / slide coordinates moved
   by 1e-6 and qvel by a relative 1e-6 -- perturbations at the level of the
@@ -89,8 +89,9 @@ def oracle_envelope(mjm, nworld, seed, nconmax_w, njmax_w, stage, fields, key=No
   Returns {field: [nvariant, ...]}.  Truncated iterative line searches and
   flat cost valleys make some worlds' results depend on rounding (the
   oracle itself takes either branch of a line-search bracket decision under
-  these perturbations); the HIP result is held to this envelope there."""
-  qpos, qvel, u = _inputs(mjm, nworld, seed, key, qpos_noise, qvel_noise, ctrl)
+  these perturbations); the HIP result is held to this envelope there.
+  `inputs` = (qpos, qvel, ctrl) replaces the seeded batch."""
+  qpos, qvel, u = inputs if inputs is not None else _inputs(mjm, nworld, seed, key, qpos_noise, qvel_noise, ctrl)
   rng = np.random.default_rng(12345)
   hinge = [mjm.jnt_qposadr[j] for j in range(mjm.njnt) if mjm.jnt_type[j] in (2, 3)]
   out = {f: [] for f in fields}
@@ -536,3 +537,67 @@ def test_support_functions():
   oracle.run("fwd_acceleration", hm, hd)
   _close("xfrc", q.cpu().numpy(), hd["qfrc_smooth"], 2e-4)
 
+
+def _n_humanoids(count):
+  """The reference's n_humanoids.xml with `count` replicas (its comment
+  invites changing the count) and a seeded batch lowered 5 cm into the
+  floor: hinges ~ N(0, 0.1^2), qvel ~ N(0, 0.3^2), ctrl ~ U(-1, 1)."""
+  from mjx_warp_amd import mjcf
+
+  xml = (models.ASSETS / "n_humanoids.xml").read_text()
+  if count != 3:
+    xml = xml.replace('<replicate count="3"', f'<replicate count="{count}"')
+  mjm = mjcf.from_xml_string(xml, base_dir=str(models.ASSETS))
+  assert mjm.nv == 27 * count
+  nworld = 8
+  rng = np.random.default_rng(31)
+  q = np.tile(mjm.qpos0, (nworld, 1))
+  for k in range(count):
+    q[:, 28 * k + 2] -= 0.05
+    q[:, 28 * k + 7:28 * (k + 1)] += rng.normal(0, 0.1, (nworld, 21))
+  v = rng.normal(0, 0.3, (nworld, mjm.nv))
+  u = rng.uniform(-1, 1, (nworld, mjm.nu))
+  return mjm, nworld, q, v, u
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_step_model_over_64k_of_lds(fused):
+  """Five attached humanoids: nv 135, 96 geoms, 130-200 constraint rows per
+  world.  The Newton Hessian alone is 71 KiB, so the forward and LDS solve
+  launches take more than 64 KiB of LDS per workgroup, which gfx950's
+  160 KiB holds.  One plain / fused step against the oracle, held to the
+  oracle's rounding envelope as test_step holds the humanoid."""
+  mjm, nworld, q, v, u = _n_humanoids(5)
+  m, d, hm, hd = _gpu_and_oracle(mjm, nworld, q, v, u, nconmax_w=96, njmax_w=256)
+  (mjw.step_fused if fused else mjw.step)(m, d)
+  torch.cuda.synchronize()
+  oracle.run("step", hm, hd)
+  rows = np.array([n for _, n in helpers.world_blocks(hd["world_efc"], nworld)])
+  assert rows.min() > 64 and rows.max() <= 256, rows
+  assert np.array_equal(rows, [n for _, n in helpers.world_blocks(helpers.data_numpy(d, "world_efc"), nworld)])
+  assert not np.any(helpers.data_numpy(d, "world_flags")), "a world was left deferred"
+  env = oracle_envelope(mjm, nworld, None, 96, 256, stage="step", fields=["qpos", "qvel", "qacc"], inputs=(q, v, u))
+  _close_or_enveloped("qpos", helpers.data_numpy(d, "qpos"), hd["qpos"], env["qpos"], 1e-4)
+  _close_or_enveloped("qvel", helpers.data_numpy(d, "qvel"), hd["qvel"], env["qvel"], 2e-3)
+  _close_or_enveloped("qacc", helpers.data_numpy(d, "qacc"), hd["qacc"], env["qacc"], 2e-3)
+
+
+def test_forward_and_step_n_humanoids():
+  """The reference's n_humanoids.xml compiled here (three attached
+  humanoids, nv 81, 64-lane teams, Newton 100 / 50 by the file's defaults),
+  lowered 5 cm into the floor: forward field by field, then a plain and a
+  fused step, against the oracle."""
+  mjm, nworld, q, v, u = _n_humanoids(3)
+  m, d, hm, hd = _gpu_and_oracle(mjm, nworld, q, v, u, nconmax_w=64, njmax_w=400)
+  mjw.forward(m, d)
+  torch.cuda.synchronize()
+  oracle.run("forward", hm, hd)
+  assert int(hd["nefc"][0]) > 0
+  compare_forward(m, d, hd)
+  for fused in (False, True):
+    m, d, hm, hd = _gpu_and_oracle(mjm, nworld, q, v, u, nconmax_w=64, njmax_w=400)
+    (mjw.step_fused if fused else mjw.step)(m, d)
+    torch.cuda.synchronize()
+    oracle.run("step", hm, hd)
+    _close(f"qpos fused={fused}", helpers.data_numpy(d, "qpos"), hd["qpos"], 1e-4)
+    _close(f"qvel fused={fused}", helpers.data_numpy(d, "qvel"), hd["qvel"], 2e-3)
