@@ -267,8 +267,10 @@ def time_heterogeneous(args, nworld: int = 16384, steps: int = 50, graph_steps: 
   hinge chain of 2-8 links plus 1-4 free spheres in a walled box, split
   into one model per (links, spheres) pair (28 models, nv 8-32), all
   advanced together: grouped (one launch per stage over every model),
-  per model (one fused step per model), and the grouped step captured as a
-  HIP graph of graph_steps steps and replayed."""
+  per model (one fused step per model), the grouped step captured as a
+  HIP graph of graph_steps steps and replayed, and the form the autotuner
+  picks (per model: lanes per world and LDS budget; buckets by LDS class or
+  one launch per stage; grouped or per model), which is `value`."""
   import numpy as np
   import torch
 
@@ -289,8 +291,17 @@ def time_heterogeneous(args, nworld: int = 16384, steps: int = 50, graph_steps: 
   ms = mjw.MultiStep(groups)
   out = {"nworld": nworld, "models": len(groups), "steps": steps,
          "nv_range": [min(int(m.nv) for m, _ in groups), max(int(m.nv) for m, _ in groups)]}
+  # every form is timed from the same state
+  state = [{f: getattr(d, f).clone() for f in ("qpos", "qvel", "qacc_warmstart")} for _, d in groups]
+
+  def restore():
+    for (_, d), st in zip(groups, state):
+      for f, v in st.items():
+        getattr(d, f).copy_(v)
+
   for mode in ("grouped", "per_group"):
     ms.mode = mode
+    restore()
     for _ in range(3):
       ms.step()
     torch.cuda.synchronize()
@@ -300,6 +311,7 @@ def time_heterogeneous(args, nworld: int = 16384, steps: int = 50, graph_steps: 
     torch.cuda.synchronize()
     out[mode] = nworld * steps / (time.perf_counter() - t0)
   ms.mode = "grouped"
+  restore()
   g = torch.cuda.CUDAGraph()
   side = torch.cuda.Stream()
   side.wait_stream(torch.cuda.current_stream())
@@ -307,6 +319,7 @@ def time_heterogeneous(args, nworld: int = 16384, steps: int = 50, graph_steps: 
     for _ in range(graph_steps):
       ms.step()
   torch.cuda.current_stream().wait_stream(side)
+  restore()
   g.replay()
   torch.cuda.synchronize()
   reps = max(1, steps // graph_steps)
@@ -315,10 +328,108 @@ def time_heterogeneous(args, nworld: int = 16384, steps: int = 50, graph_steps: 
     g.replay()
   torch.cuda.synchronize()
   out["grouped_graph"] = nworld * reps * graph_steps / (time.perf_counter() - t0)
-  out["value"] = max(out["grouped"], out["per_group"], out["grouped_graph"])
+  ms.close()
+  t0 = time.perf_counter()
+  restore()
+  tuned = mjw.MultiStep(groups, autotune=True)
+  out["autotune_s"] = time.perf_counter() - t0
+  restore()
+  for _ in range(3):
+    tuned.step()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(steps):
+    tuned.step()
+  torch.cuda.synchronize()
+  out["autotuned"] = nworld * steps / (time.perf_counter() - t0)
+  teams = {}
+  for t in tuned.tunes or []:
+    teams[f"team{t[0]}_rj{t[2]}"] = teams.get(f"team{t[0]}_rj{t[2]}", 0) + 1
+  out["autotuned_choice"] = {"mode": tuned.mode, "buckets": tuned.buckets, "per_model": teams}
+  out["value"] = out["autotuned"]
   out["unit"] = "world-steps/s"
   out["finite"] = bool(all(torch.isfinite(d.qpos).all().item() for _, d in groups))
+  tuned.close()
+  return out
+
+
+def time_heterogeneous_stated(args, nworld: int = 16384, steps: int = 100):
+  """World-steps/s of the heterogeneous workload at its stated size: 16,384
+  worlds, each a hinge chain of U{2..32} 0.1 m capsule links (folded inside
+  the box) plus U{1..64} free spheres (radius U(0.03, 0.1), density 1000) in
+  a 1 m box of 5 planes, Newton -- one model per (links, spheres) pair
+  present, nv 8 to 416 -- advanced together by the grouped step (with and
+  without buckets by LDS class, the faster kept), `steps` steps captured as
+  one HIP graph and replayed (the value), and eagerly."""
+  import numpy as np
+  import torch
+
+  import mjx_warp_amd as mjw
+  from mjx_warp_amd import models
+
+  t0 = time.perf_counter()
+  spec = models.heterogeneous_groups(nworld, max_links=32, max_spheres=64)
+  mjms = models.box_worlds(spec, processes=min(16, os.cpu_count() or 1))
+  t_compile = time.perf_counter() - t0
+  rng = np.random.default_rng(2)
+  groups = []
+  for (links, radii, n), mjm in zip(spec, mjms):
+    q, _ = models.box_world_states(mjm, links, radii, n, rng)
+    m = mjw.put_model(mjm)
+    ncon_w = 3 * (links + len(radii))
+    d = mjw.make_data(mjm, nworld=n, nconmax=ncon_w * n, njmax=(4 * ncon_w + mjm.njnt) * n)
+    d.qpos.copy_(torch.tensor(q, dtype=torch.float32))
+    groups.append((m, d))
+  torch.cuda.synchronize()
+  t_setup = time.perf_counter() - t0
+  log(f"stated-size heterogeneous: {len(groups)} models, setup {t_setup:.1f} s (compile {t_compile:.1f} s)")
+  state = [{f: getattr(d, f).clone() for f in ("qpos", "qvel", "qacc_warmstart")} for _, d in groups]
+
+  def restore():
+    for (_, d), st in zip(groups, state):
+      for f, v in st.items():
+        getattr(d, f).copy_(v)
+
+  out = {"nworld": nworld, "models": len(groups), "steps": steps, "setup_s": t_setup,
+         "nv_range": [min(int(m.nv) for m, _ in groups), max(int(m.nv) for m, _ in groups)]}
+  best = None
+  for buckets in (False, True):
+    ms = mjw.MultiStep(groups, buckets=buckets)
+    restore()
+    ms.step()
+    torch.cuda.synchronize()
+    restore()
+    t1 = time.perf_counter()
+    for _ in range(10):
+      ms.step()
+    torch.cuda.synchronize()
+    rate = nworld * 10 / (time.perf_counter() - t1)
+    out["grouped_buckets" if buckets else "grouped"] = rate
+    if best is None or rate > best[0]:
+      best = (rate, buckets)
+    ms.close()
+  ms = mjw.MultiStep(groups, buckets=best[1])
+  restore()
+  g = torch.cuda.CUDAGraph()
+  side = torch.cuda.Stream()
+  side.wait_stream(torch.cuda.current_stream())
+  with torch.cuda.graph(g, stream=side):
+    for _ in range(steps):
+      ms.step()
+  torch.cuda.current_stream().wait_stream(side)
+  restore()
+  torch.cuda.synchronize()
+  t1 = time.perf_counter()
+  g.replay()
+  torch.cuda.synchronize()
+  out["graph_rollout"] = nworld * steps / (time.perf_counter() - t1)
+  out["buckets"] = best[1]
+  out["value"] = out["graph_rollout"]
+  out["unit"] = "world-steps/s"
+  out["finite"] = bool(all(torch.isfinite(d.qpos).all().item() for _, d in groups))
+  out["rows_per_world"] = float(sum(int(d.nefc[0].item()) for _, d in groups)) / nworld
   ms.close()
+  del g
   return out
 
 
@@ -396,6 +507,8 @@ def main():
   ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint rollout")
   ap.add_argument("--no-rollout-graph", action="store_true", help="skip the graph-replayed rollout")
   ap.add_argument("--no-hetero", action="store_true", help="skip the heterogeneous (many-model) batch")
+  ap.add_argument("--no-hetero-stated", action="store_true",
+                  help="skip the heterogeneous batch at its stated size (chains to 32 links, to 64 spheres)")
   ap.add_argument("--no-fallen", action="store_true", help="skip the fallen-humanoid leg")
   ap.add_argument("--fallen-preroll", type=int, default=1000, help="untimed steps before the fallen leg")
   ap.add_argument("--graph", action="store_true", help="replay a HIP graph of one captured step")
@@ -445,7 +558,15 @@ def main():
   if not args.no_hetero:
     het = time_heterogeneous(args)
     log(f"heterogeneous: grouped {het['grouped']:.4g}, per model {het['per_group']:.4g}, "
-        f"grouped graph {het['grouped_graph']:.4g} world-steps/s")
+        f"grouped graph {het['grouped_graph']:.4g}, autotuned {het['autotuned']:.4g} world-steps/s "
+        f"({het['autotuned_choice']}, tuning {het['autotune_s']:.1f} s)")
+  het_stated = None
+  if not args.no_hetero_stated:
+    het_stated = time_heterogeneous_stated(args)
+    log(f"heterogeneous at stated size: {het_stated['models']} models, nv {het_stated['nv_range']}, "
+        f"{het_stated['rows_per_world']:.1f} rows per world: grouped {het_stated['grouped']:.4g}, "
+        f"buckets {het_stated['grouped_buckets']:.4g}, {het_stated['steps']}-step graph rollout "
+        f"{het_stated['graph_rollout']:.4g} world-steps/s")
   adj = None
   if not args.no_adjoint:
     adj = time_adjoint(args)
@@ -494,6 +615,7 @@ def main():
     "rollout_graph": rgraph,
     "forward_adjoint": adj,
     "heterogeneous": het,
+    "heterogeneous_stated_size": het_stated,
   }
   print(json.dumps(out), flush=True)
 

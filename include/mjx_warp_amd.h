@@ -90,7 +90,7 @@ typedef struct mjhOption {
   X(nq) X(nv) X(nu) X(na) X(nbody) X(njnt) X(ngeom) X(nsite) X(nexclude) X(nmocap) \
   X(nM) X(nlsp) X(nlevel) X(nlimited) X(ntril) X(nfullm) X(nmulm) X(nqLD_update) \
   X(nqLD_update_level) X(nqLD_tile) X(nqLD_group) X(nact_tile) X(nact_group) \
-  X(nmesh) X(nmeshvert) X(actuator_affine_bias_gain) X(max_level_width) X(nnxn)
+  X(nmesh) X(nmeshvert) X(actuator_affine_bias_gain) X(max_level_width) X(nnxn) X(tree_dof_max)
 
 #define MJH_MODEL_ARRAYS(X) \
   X(R, qpos0, (nq)) \
@@ -480,6 +480,29 @@ typedef struct mjhMulti mjhMulti;
  * here that allocates: the group table in device memory).  The plan holds
  * the groups' pointers: their buffers must outlive it. */
 int mjh_multi_create(int ngroups, const mjhModel* models, const mjhData* datas, mjhMulti** out);
+/* Launch choices for one group of a plan (no counterpart in the reference).
+ * team: lanes per world -- 32 (two worlds per wavefront, register solver,
+ * nv <= 32 only) or 64 (one world per wavefront, LDS solver); 0 takes the
+ * library's choice (32 iff nv <= 32).  rmax: the LDS solver's row capacity
+ * per world (a world with more rows is solved by the global-memory tail);
+ * <= 0 takes the library's.  rj: Jacobian rows per world staged in LDS
+ * (all of a world's rows when it has at most rj, else none on 64 lanes; the
+ * first rj on 32 lanes); < 0 takes the library's.  Every choice computes
+ * the same step; they differ in speed. */
+typedef struct mjhGroupTune {
+  int team;
+  int rmax;
+  int rj;
+} mjhGroupTune;
+/* flags of mjh_multi_create_tuned: launch the groups in buckets, one per
+ * team size and LDS occupancy class (workgroups a CU holds by the LDS of the
+ * group's launches), each bucket's launches sized for its own groups; without
+ * it every launch takes the largest group's LDS. */
+#define MJH_MULTI_BUCKETS 1
+/* mjh_multi_create with per-group choices (`tunes`: ngroups entries, or
+ * NULL for the library's) and flags. */
+int mjh_multi_create_tuned(int ngroups, const mjhModel* models, const mjhData* datas, const mjhGroupTune* tunes,
+                           int flags, mjhMulti** out);
 /* One fused step (mjh_step_fused) of every world of every group. */
 int mjh_multi_step(const mjhMulti* plan, mjhStream stream);
 /* Frees the plan (after its last step has finished). */

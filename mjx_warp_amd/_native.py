@@ -91,6 +91,15 @@ def step_desc_words() -> int:
   return int(re.search(r"#define\s+MJH_STEP_DESC_WORDS\s+(\d+)", HEADER.read_text()).group(1))
 
 
+class GroupTuneStruct(ctypes.Structure):
+  """mjhGroupTune: lanes per world (0, 32, 64), LDS-solver row capacity
+  (<= 0: the library's), Jacobian rows staged in LDS (< 0: the library's)."""
+  _fields_ = [("team", ctypes.c_int), ("rmax", ctypes.c_int), ("rj", ctypes.c_int)]
+
+
+MJH_MULTI_BUCKETS = 1
+
+
 def model_sizes(m) -> dict:
   msizes, _ = layout()["model"]
   return {s: int(getattr(m, s)) for s in msizes}
@@ -202,6 +211,7 @@ def lib():
   L.mjh_xfrc_accumulate.argtypes = [P, P, P, P]
   L.mjh_rollout.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P]
   L.mjh_multi_create.argtypes = [ctypes.c_int, P, P, P]
+  L.mjh_multi_create_tuned.argtypes = [ctypes.c_int, P, P, P, ctypes.c_int, P]
   L.mjh_multi_step.argtypes = [P, P]
   L.mjh_multi_step_static.argtypes = [ctypes.c_int, P, P, P]
   L.mjh_multi_destroy.argtypes = [P]

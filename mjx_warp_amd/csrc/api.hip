@@ -11,6 +11,7 @@
 // step, the integrator).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -218,18 +219,65 @@ __global__ void __launch_bounds__(64) k_adjoint_state_grad(mjhModel m, mjhData d
   adjoint_state_grad(m, d, lam, g_qpos, g_qvel, g_act, t);
 }
 
-int lds_linalg(const mjhModel& m) { return m.nv * m.nv + m.nv; }
+// per-block linear algebra (factor_m, solve_m, the integrators): the
+// largest kinematic tree's block and right-hand side staged in LDS, and
+// MuJoCo's sparse factor (nM) for sparse models
+int lds_linalg(const mjhModel& m) {
+  const int b = m.tree_dof_max > 0 ? m.tree_dof_max : m.nv;
+  const int a = b * b + b;
+  return m.opt.is_sparse && m.nM > a ? m.nM : a;
+}
+// the global-memory solver of the tail launches: per-block linear algebra
+// and its Newton Hessian with right-hand side (nv^2 + nv), or only the
+// right-hand side when wide (H in efc_h)
+int lds_tail(const mjhModel& m, bool wide) {
+  const int a = lds_linalg(m), h = wide ? m.nv : m.nv * m.nv + m.nv;
+  return a > h ? a : h;
+}
 int lds_sap(const mjhModel& m) {
   int np = 1;
   while (np < m.ngeom) np <<= 1;
   return 2 * np + 4 + 4 * m.ngeom;
 }
 
+size_t lds_cap();
+bool reg_path(const mjhModel& m);
+int lds_forward(const mjhModel& m, bool reg);
+constexpr int kLdsRowsMin = 256, kLdsRowsMax = 1024;
+
 // Row capacity of the LDS solver (rows of one world) and how many of them
-// get their Jacobian rows staged in LDS; MJH_SOLVER_RMAX / MJH_SOLVER_RJ
-// override the defaults.
-Tune tune_for(const mjhModel& m) {
-  Tune t{64, 32};
+// get their Jacobian rows staged in LDS, for a model on 32-lane (reg) or
+// 64-lane teams.  On the 64-lane path no rows are staged (the LDS solver
+// stages a world's rows only when all of them fit, and the LDS that takes
+// costs occupancy: the three attached humanoids step 1.25x faster without
+// it); the Newton Hessian moves to efc_h (wide) when with it in LDS next to
+// 256 rows a CU would hold a single workgroup (five attached humanoids,
+// nv 135, step 2.2x faster wide); and the row capacity fills what
+// the solver's LDS leaves before a CU would hold one workgroup fewer (a CU
+// holds lds_cap() bytes: 160 KiB on gfx950): rows cost 8 floats each, and a
+// world with more rows than the capacity is solved by the slower
+// global-memory tail.  MJH_SOLVER_RMAX / MJH_SOLVER_RJ override the
+// defaults, MJH_SOLVER_WIDE=1 takes the Hessian to efc_h for any 64-lane
+// model.
+Tune tune_for(const mjhModel& m, bool reg) {
+  Tune t{64, reg ? 32 : 0, 0};
+  if (!reg) {
+    t.rmax = kLdsRowsMin;
+    const int cap = (int)(lds_cap() / sizeof(float));
+    const char* force_wide = std::getenv("MJH_SOLVER_WIDE");
+    t.wide = lds_solver_floats(m.nv, kLdsRowsMin, t.rj) > cap / 2 || lds_tail(m, false) > cap ||
+             (force_wide && std::strcmp(force_wide, "1") == 0);
+    int base = lds_solver_floats(m.nv, 0, t.rj, t.wide);
+    const int fwd = lds_forward(m, false);
+    if (fwd > base) base = fwd;
+    if (base + 8 * kLdsRowsMin <= cap) {
+      // wide worlds are few per CU anyway and a row past the capacity sends
+      // the world to the slow tail: they take rows up to half the LDS
+      const int per_cu = t.wide ? 2 : cap / (base + 8 * kLdsRowsMin);
+      const int rows = (cap / per_cu - lds_solver_floats(m.nv, 0, t.rj, t.wide)) / 8;
+      t.rmax = rows < kLdsRowsMin ? kLdsRowsMin : rows > kLdsRowsMax ? kLdsRowsMax : rows;
+    }
+  }
   if (const char* e = std::getenv("MJH_SOLVER_RMAX")) t.rmax = std::atoi(e);
   if (const char* e = std::getenv("MJH_SOLVER_RJ")) t.rj = std::atoi(e);
   if (t.rmax < 0) t.rmax = 0;
@@ -237,31 +285,42 @@ Tune tune_for(const mjhModel& m) {
   if (t.rj < 0) t.rj = 0;
   return t;
 }
-int lds_solve(const mjhModel& m) {
-  const Tune t = tune_for(m);
-  const int a = lds_solver_floats(m.nv, t.rmax, t.rj), b = lds_linalg(m);
+Tune tune_for(const mjhModel& m) { return tune_for(m, reg_path(m)); }
+int lds_solve(const mjhModel& m, const Tune& t) {
+  const int a = lds_solver_floats(m.nv, t.rmax, t.rj, t.wide != 0), b = lds_linalg(m);
   return a > b ? a : b;
 }
+int lds_solve(const mjhModel& m) { return lds_solve(m, tune_for(m)); }
 int lds_kin(const mjhModel& m) { return lds_kinematics_floats(m.nbody); }
-int lds_crb(const mjhModel& m) { return lds_crb_floats(m.nbody, m.nv); }
+int lds_crb(const mjhModel& m, bool reg) { return lds_crb_floats(m.nbody, m.nv, reg); }
 int lds_vel(const mjhModel& m) { return lds_velocity_floats(m.nbody); }
 int lds_broadphase(const mjhModel& m) { return m.ngeom > 100 ? lds_sap(m) : kNxnWords * m.ngeom; }
-bool reg_path(const mjhModel& m);
-int lds_con(const mjhModel& m) {
-  return reg_path(m) ? lds_constraint_floats<kRegTeam>() : lds_constraint_floats<kTeam>();
-}
-// every forward stage of the fused forward launch
-int lds_forward(const mjhModel& m) {
+int lds_con(bool reg) { return reg ? lds_constraint_floats<kRegTeam>() : lds_constraint_floats<kTeam>(); }
+// every forward stage of the fused forward launch, on 32-lane (reg) or
+// 64-lane teams
+int lds_forward(const mjhModel& m, bool reg) {
   int a = lds_kin(m);
-  const int b = lds_crb(m), c = lds_vel(m), g = lds_broadphase(m), k = lds_con(m);
+  const int b = lds_crb(m, reg), c = lds_vel(m), g = lds_broadphase(m), k = lds_con(reg);
   a = a > b ? a : b;
   a = a > c ? a : c;
   a = a > k ? a : k;
   return a > g ? a : g;
 }
-int lds_step(const mjhModel& m) {
-  const int a = lds_solve(m), b = lds_forward(m);
+int lds_forward(const mjhModel& m) { return lds_forward(m, reg_path(m)); }
+// the 64-lane fused step (forward, LDS solver, integrator in one launch)
+int lds_step(const mjhModel& m, const Tune& t) {
+  const int a = lds_solve(m, t), b = lds_forward(m, false);
   return a > b ? a : b;
+}
+int lds_step(const mjhModel& m) { return lds_step(m, tune_for(m, false)); }
+// The 64-lane fused step as two launches (forward; solve and integrator),
+// each with its own LDS, or as one launch (MJH_FUSED64=one).
+bool split64() {
+  static const bool one = [] {
+    const char* e = std::getenv("MJH_FUSED64");
+    return e && std::strcmp(e, "one") == 0;
+  }();
+  return !one;
 }
 
 // Worlds of models with nv <= 32 run on 32-lane teams (two per wavefront)
@@ -276,23 +335,25 @@ bool reg_path(const mjhModel& m) {
 }
 // per-world LDS of the 32-lane kernels, a multiple of 4 floats so every
 // world's broadcast buffers are float4-aligned
-int lds_solve_reg(const mjhModel& m) {
-  const int a = lds_reg_floats(tune_for(m).rj, (m.nv + 3) & ~3), b = lds_linalg(m);
+int lds_solve_reg(const mjhModel& m, const Tune& t) {
+  const int a = lds_reg_floats(t.rj, (m.nv + 3) & ~3), b = lds_linalg(m);
   return ((a > b ? a : b) + 3) & ~3;
 }
+int lds_solve_reg(const mjhModel& m) { return lds_solve_reg(m, tune_for(m)); }
 // The register tail stages as many Jacobian rows as two worlds' LDS fit in
 // 64 KB (up to every row of kRegRows).
-Tune tune_tail_reg(const mjhModel& m) {
-  Tune t = tune_for(m);
+Tune tune_tail_reg(const mjhModel& m, Tune t) {
   const int nvc = (m.nv + 3) & ~3;
   const int fit = (16384 / 2 - lds_reg_floats(0, nvc) - 3) / (nvc + 1);
   t.rj = fit < kRegRows ? fit : kRegRows;
   return t;
 }
-int lds_tail_reg(const mjhModel& m) {
-  const int a = lds_reg_floats(tune_tail_reg(m).rj, (m.nv + 3) & ~3), b = lds_linalg(m);
+Tune tune_tail_reg(const mjhModel& m) { return tune_tail_reg(m, tune_for(m)); }
+int lds_tail_reg(const mjhModel& m, const Tune& tail) {
+  const int a = lds_reg_floats(tail.rj, (m.nv + 3) & ~3), b = lds_linalg(m);
   return ((a > b ? a : b) + 3) & ~3;
 }
+int lds_tail_reg(const mjhModel& m) { return lds_tail_reg(m, tune_tail_reg(m)); }
 
 // A second stream per device and host thread for work that may run beside
 // the caller's stream (default priority; ordered against it with events).
@@ -455,19 +516,19 @@ int do_solve(const mjhModel& m, const mjhData& d, hipStream_t s) {
   } else {
     TRY((launch_team<StSolveTeam, kTeam>(m, d, s, lds_solve(m), "solve")));
   }
-  return launch_team<StTail<false>, kTeam>(m, d, s, lds_linalg(m), "solve_tail");
+  return launch_team<StTail<false>, kTeam>(m, d, s, lds_tail(m, tune_for(m).wide != 0), "solve_tail");
 }
 
 int do_fwd_position(const mjhModel& m, const mjhData& d, hipStream_t s) {
   TRY(launch<StKinematics>(m, d, s, lds_kin(m), "kinematics"));
   TRY(launch<StComPos>(m, d, s, lds_kin(m), "com_pos"));
-  TRY(launch<StCrb>(m, d, s, lds_crb(m), "crb"));
-  TRY(launch<StFactorM>(m, d, s, m.nv * m.nv, "factor_m"));
+  TRY(launch<StCrb>(m, d, s, lds_crb(m, reg_path(m)), "crb"));
+  TRY(launch<StFactorM>(m, d, s, lds_linalg(m), "factor_m"));
   TRY(reset(d, s, 1));
   TRY(do_collision(m, d, s));
   if (!(m.opt.disableflags & mjhDSBL_CONSTRAINT)) {
     TRY(reset(d, s, 2));
-    TRY(launch<StMakeConstraint>(m, d, s, lds_con(m), "make_constraint"));
+    TRY(launch<StMakeConstraint>(m, d, s, lds_con(reg_path(m)), "make_constraint"));
   }
   return launch<StTransmission>(m, d, s, 0, "transmission");
 }
@@ -517,7 +578,7 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
   } else {
     TRY((launch_team<StFusedStep, kTeam>(m, d, s, lds_step(m), "step_fused")));
   }
-  return launch_team<StTail<true>, kTeam>(m, d, s, lds_linalg(m), "step_tail");
+  return launch_team<StTail<true>, kTeam>(m, d, s, lds_tail(m, tune_for(m).wide != 0), "step_tail");
 }
 
 // one workgroup per group
@@ -536,12 +597,23 @@ struct GroupTable {
 
 }  // namespace
 
-// The plan of a heterogeneous batch: device tables of its groups (32-lane
-// groups: nv <= 32, seen by the 32-lane launches and by the 64-lane tail;
-// 64-lane groups) and each launch's per-world LDS stride.
-struct mjhMulti {
+namespace {
+
+// Groups launched together: device tables of its groups (32-lane groups,
+// seen by the 32-lane launches and by the 64-lane tail; 64-lane groups) and
+// each launch's per-world LDS stride, the largest over its groups.
+struct Bucket {
   GroupTable t32, t32_64, t64;
   int lds_fwd = 0, lds_solve = 0, lds_big = 0, lds_tail32 = 0, lds_step = 0, lds_tail64 = 0;
+  int lds_fwd64 = 0, lds_solve64 = 0;
+};
+
+}  // namespace
+
+// The plan of a heterogeneous batch: one bucket, or (MJH_MULTI_BUCKETS) one
+// per team size and LDS occupancy class, each with its own launches.
+struct mjhMulti {
+  std::vector<Bucket> buckets;
 };
 
 namespace {
@@ -562,8 +634,9 @@ int build_table(const std::vector<GroupDesc>& host, int wpb, GroupTable* t) {
 }
 
 void free_tables(mjhMulti* p) {
-  for (GroupTable* t : {&p->t32, &p->t32_64, &p->t64})
-    if (t->dev) (void)hipFree(t->dev);
+  for (Bucket& b : p->buckets)
+    for (GroupTable* t : {&b.t32, &b.t32_64, &b.t64})
+      if (t->dev) (void)hipFree(t->dev);
 }
 
 template <int TEAM, class S, int TI>
@@ -633,9 +706,10 @@ int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
   GroupDesc* dev = reinterpret_cast<GroupDesc*>(d.step_desc);
   const bool reg = reg_path(m);
   const int l_fwd = lds_forward_reg(m), l_solve = lds_solve_reg(m), l_big = lds_tail_reg(m), l_step = lds_step(m),
-            l_tail = lds_linalg(m);
+            l_tail = lds_tail(m, g.tune[0].wide != 0), l_fwd64 = lds_forward(m, false),
+            l_solve64 = lds_solve(m, g.tune[0]);
   const size_t need = reg ? (size_t)max_of(max_of(2 * l_fwd, 2 * l_solve), max_of(2 * l_big, l_tail))
-                          : (size_t)max_of(l_step, l_tail);
+                          : (size_t)max_of(split64() ? max_of(l_fwd64, l_solve64) : l_step, l_tail);
   if (need * sizeof(float) > lds_cap())
     return fail(MJH_ERR_CAPACITY,
                 "step_fused: model too large for the dense LDS kernels (nv = " + std::to_string(m.nv) + ")");
@@ -653,51 +727,111 @@ int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
         [&](hipStream_t q) {
           return launch_groups<kRegTeam, StFusedSolveIntegrate, 0>(t32, l_solve, q, "step_fused_solve");
         }));
+  } else if (split64()) {
+    TRY((launch_groups<kTeam, StFusedForward, 0>(t64, l_fwd64, s, "step_fused_forward")));
+    TRY((launch_groups<kTeam, StFusedSolveIntegrate, 0>(t64, l_solve64, s, "step_fused_solve")));
   } else {
     TRY((launch_groups<kTeam, StFusedStep, 0>(t64, l_step, s, "step_fused")));
   }
   return launch_groups<kTeam, StTail<true>, 0>(t64, l_tail, s, "step_tail");
 }
 
-// Checks group i (as mjh_check and the fused step do), fills its table
-// entry (model, data, tuning; first_block is the caller's) and folds its
-// per-world LDS strides into plan.  reg: the group runs on 32-lane teams.
-int plan_group(int i, const mjhModel& m, const mjhData& d, mjhMulti* plan, GroupDesc* gd, bool* reg) {
+// How one model's worlds are launched: on 32-lane teams with the register
+// solver (reg) or on 64-lane teams with the LDS solver, and the solver's
+// tuning for the solve / tail launches ([0]) and the big-world launch ([1]);
+// the library's choice, or the caller's where o sets it.
+struct Choice {
+  bool reg = false;
+  Tune tune[2] = {};
+};
+int choose(int i, const mjhModel& m, const mjhGroupTune* o, Choice* c) {
+  const std::string g = "multi: group " + std::to_string(i) + ": ";
+  c->reg = reg_path(m);
+  if (o) {
+    if (o->team != 0 && o->team != kRegTeam && o->team != kTeam) return fail(MJH_ERR_INVALID, g + "team must be 0, 32 or 64");
+    if (o->team == kRegTeam && m.nv > kRegNvMax) return fail(MJH_ERR_INVALID, g + "32-lane teams need nv <= 32");
+    if (o->team) c->reg = o->team == kRegTeam;
+  }
+  Tune t = tune_for(m, c->reg);
+  if (o && o->rmax > 0) t.rmax = o->rmax;
+  if (o && o->rj >= 0) t.rj = o->rj;
+  if (t.rj > t.rmax) t.rj = t.rmax;
+  c->tune[0] = t;
+  c->tune[1] = c->reg ? tune_tail_reg(m, t) : t;
+  return MJH_OK;
+}
+
+// per-world LDS strides of a group's launches
+struct GroupLds {
+  int fwd = 0, solve = 0, big = 0, tail32 = 0, step = 0, tail64 = 0, fwd64 = 0, solve64 = 0;
+};
+GroupLds group_lds(const mjhModel& m, const Choice& c) {
+  GroupLds l;
+  if (c.reg) {
+    l.fwd = reg_stride(lds_forward(m, true));
+    l.solve = lds_solve_reg(m, c.tune[0]);
+    l.big = lds_tail_reg(m, c.tune[1]);
+    l.tail32 = lds_tail(m, false);
+  } else {
+    l.step = lds_step(m, c.tune[0]);
+    l.tail64 = lds_tail(m, c.tune[0].wide != 0);
+    l.fwd64 = lds_forward(m, false);
+    l.solve64 = lds_solve(m, c.tune[0]);
+  }
+  return l;
+}
+void fold(Bucket* b, const GroupLds& l) {
+  b->lds_fwd = max_of(b->lds_fwd, l.fwd);
+  b->lds_solve = max_of(b->lds_solve, l.solve);
+  b->lds_big = max_of(b->lds_big, l.big);
+  b->lds_tail32 = max_of(b->lds_tail32, l.tail32);
+  b->lds_step = max_of(b->lds_step, l.step);
+  b->lds_tail64 = max_of(b->lds_tail64, l.tail64);
+  b->lds_fwd64 = max_of(b->lds_fwd64, l.fwd64);
+  b->lds_solve64 = max_of(b->lds_solve64, l.solve64);
+}
+// Workgroups a CU holds by the LDS of the group's main launches (the
+// forward and solve launches), at most 16.
+int lds_class(const Choice& c, const GroupLds& l) {
+  const size_t words = c.reg ? 2 * (size_t)max_of(l.fwd, max_of(l.solve, l.big))
+                             : (size_t)(split64() ? max_of(l.fwd64, l.solve64) : l.step);
+  const size_t bytes = words * sizeof(float);
+  const size_t k = bytes ? lds_cap() / bytes : 16;
+  return k < 16 ? (int)k : 16;
+}
+
+// Checks group i (as mjh_check and the fused step do) and fills its table
+// entry (model, data, tuning; first_block is the caller's), its launch
+// choice and its LDS strides.
+int plan_group(int i, const mjhModel& m, const mjhData& d, const mjhGroupTune* o, GroupDesc* gd, Choice* c,
+               GroupLds* l) {
   if (mjh_check(&m, &d) != MJH_OK) return fail(MJH_ERR_INVALID, "multi: group " + std::to_string(i) + ": " + g_last_error);
   if (m.opt.integrator != mjhINT_EULER && m.opt.integrator != mjhINT_IMPLICITFAST)
     return fail(MJH_ERR_UNSUPPORTED, "multi: group " + std::to_string(i) + ": integrator " +
                                          std::to_string(m.opt.integrator) + " not implemented.");
   TRY(check_implicit_dense(m));
+  TRY(choose(i, m, o, c));
   gd->m = m;
   gd->d = d;
   gd->first_block = 0;
-  gd->tune[0] = tune_for(m);
-  *reg = reg_path(m);
-  if (*reg) {
-    gd->tune[1] = tune_tail_reg(m);
-    plan->lds_fwd = max_of(plan->lds_fwd, lds_forward_reg(m));
-    plan->lds_solve = max_of(plan->lds_solve, lds_solve_reg(m));
-    plan->lds_big = max_of(plan->lds_big, lds_tail_reg(m));
-    plan->lds_tail32 = max_of(plan->lds_tail32, lds_linalg(m));
-  } else {
-    gd->tune[1] = gd->tune[0];
-    plan->lds_step = max_of(plan->lds_step, lds_step(m));
-    plan->lds_tail64 = max_of(plan->lds_tail64, lds_linalg(m));
-  }
+  gd->tune[0] = c->tune[0];
+  gd->tune[1] = c->tune[1];
+  *l = group_lds(m, *c);
   return MJH_OK;
 }
 
 // each launch's LDS: two worlds per workgroup on 32-lane teams, one on 64
-int plan_capacity(const mjhMulti& plan) {
+int plan_capacity(const Bucket& plan) {
   const int need = max_of(max_of(2 * plan.lds_fwd, 2 * plan.lds_solve), max_of(2 * plan.lds_big, plan.lds_tail32));
-  if ((size_t)max_of(need, max_of(plan.lds_step, plan.lds_tail64)) * sizeof(float) > lds_cap())
+  const int need64 = split64() ? max_of(plan.lds_fwd64, plan.lds_solve64) : plan.lds_step;
+  if ((size_t)max_of(need, max_of(need64, plan.lds_tail64)) * sizeof(float) > lds_cap())
     return fail(MJH_ERR_CAPACITY, "multi: a group's model is too large for the dense LDS kernels");
   return MJH_OK;
 }
 
 // The launches of one fused step of every group of a plan; the big-world
 // launch beside the solve launch when use_side, else in order on s.
-int multi_launches(const mjhMulti& p, hipStream_t s, bool use_side) {
+int multi_launches(const Bucket& p, hipStream_t s, bool use_side) {
   for (const GroupTable* t : {&p.t32, &p.t64}) {
     if (t->ngroups == 0) continue;
     hipLaunchKernelGGL(k_reset_groups, dim3(t->ngroups), dim3(64), 0, s, t->dev);
@@ -717,7 +851,12 @@ int multi_launches(const mjhMulti& p, hipStream_t s, bool use_side) {
     TRY((launch_groups<kTeam, StTail<true>, 0>(p.t32_64, p.lds_tail32, s, "multi: tail")));
   }
   if (p.t64.ngroups) {
-    TRY((launch_groups<kTeam, StFusedStep, 0>(p.t64, p.lds_step, s, "multi: step")));
+    if (split64()) {
+      TRY((launch_groups<kTeam, StFusedForward, 0>(p.t64, p.lds_fwd64, s, "multi: forward")));
+      TRY((launch_groups<kTeam, StFusedSolveIntegrate, 0>(p.t64, p.lds_solve64, s, "multi: solve")));
+    } else {
+      TRY((launch_groups<kTeam, StFusedStep, 0>(p.t64, p.lds_step, s, "multi: step")));
+    }
     TRY((launch_groups<kTeam, StTail<true>, 0>(p.t64, p.lds_tail64, s, "multi: tail")));
   }
   return MJH_OK;
@@ -781,8 +920,8 @@ int mjh_check(const mjhModel* m, const mjhData* d) {
 
 STAGE_ENTRY(mjh_kinematics, StKinematics, lds_kin(*m))
 STAGE_ENTRY(mjh_com_pos, StComPos, lds_kin(*m))
-STAGE_ENTRY(mjh_crb, StCrb, lds_crb(*m))
-STAGE_ENTRY(mjh_factor_m, StFactorM, m->nv * m->nv)
+STAGE_ENTRY(mjh_crb, StCrb, lds_crb(*m, reg_path(*m)))
+STAGE_ENTRY(mjh_factor_m, StFactorM, lds_linalg(*m))
 STAGE_ENTRY(mjh_rne, StRne, lds_vel(*m))
 STAGE_ENTRY(mjh_transmission, StTransmission, 0)
 STAGE_ENTRY(mjh_com_vel, StComVel, lds_vel(*m))
@@ -828,7 +967,7 @@ int mjh_collision(const mjhModel* m, const mjhData* d, mjhStream stream) {
 int mjh_make_constraint(const mjhModel* m, const mjhData* d, mjhStream stream) {
   if (m->opt.disableflags & mjhDSBL_CONSTRAINT) return MJH_OK;
   TRY(reset(*d, (hipStream_t)stream, 2));
-  return launch<StMakeConstraint>(*m, *d, (hipStream_t)stream, lds_con(*m), "make_constraint");
+  return launch<StMakeConstraint>(*m, *d, (hipStream_t)stream, lds_con(reg_path(*m)), "make_constraint");
 }
 int mjh_fwd_position(const mjhModel* m, const mjhData* d, mjhStream stream) {
   return do_fwd_position(*m, *d, (hipStream_t)stream);
@@ -879,22 +1018,49 @@ int mjh_xfrc_accumulate(const mjhModel* m, const mjhData* d, float* qfrc, mjhStr
 }
 
 int mjh_multi_create(int ngroups, const mjhModel* models, const mjhData* datas, mjhMulti** out) {
+  return mjh_multi_create_tuned(ngroups, models, datas, nullptr, 0, out);
+}
+int mjh_multi_create_tuned(int ngroups, const mjhModel* models, const mjhData* datas, const mjhGroupTune* tunes,
+                           int flags, mjhMulti** out) {
   if (!out) return fail(MJH_ERR_INVALID, "multi: null output");
   *out = nullptr;
   if (ngroups <= 0 || !models || !datas) return fail(MJH_ERR_INVALID, "multi: no groups");
-  std::vector<GroupDesc> g32, g64;
-  mjhMulti plan;
+  if (flags & ~MJH_MULTI_BUCKETS) return fail(MJH_ERR_INVALID, "multi: unknown flags");
+  std::vector<Bucket> buckets;
+  std::vector<std::vector<GroupDesc>> g32, g64;
+  std::map<std::pair<int, int>, int> index;
   for (int i = 0; i < ngroups; i++) {
     GroupDesc gd{};
-    bool reg = false;
-    TRY(plan_group(i, models[i], datas[i], &plan, &gd, &reg));
-    (reg ? g32 : g64).push_back(gd);
+    Choice c;
+    GroupLds l;
+    TRY(plan_group(i, models[i], datas[i], tunes ? tunes + i : nullptr, &gd, &c, &l));
+    const std::pair<int, int> key = (flags & MJH_MULTI_BUCKETS) ? std::make_pair(c.reg ? 1 : 0, lds_class(c, l))
+                                                                : std::make_pair(0, 0);
+    auto it = index.find(key);
+    if (it == index.end()) {
+      it = index.emplace(key, (int)buckets.size()).first;
+      buckets.emplace_back();
+      g32.emplace_back();
+      g64.emplace_back();
+    }
+    fold(&buckets[it->second], l);
+    (c.reg ? g32 : g64)[it->second].push_back(gd);
   }
-  TRY(plan_capacity(plan));
-  mjhMulti* p = new mjhMulti(plan);
-  int rc = build_table(g32, 2, &p->t32);
-  if (rc == MJH_OK) rc = build_table(g32, 1, &p->t32_64);
-  if (rc == MJH_OK) rc = build_table(g64, 1, &p->t64);
+  for (const Bucket& b : buckets) TRY(plan_capacity(b));
+  // largest models first: the hardware hands out workgroups in index
+  // order, so the longest worlds start first and the short ones fill the
+  // end of the launch (results do not depend on the order)
+  const auto larger = [](const GroupDesc& a, const GroupDesc& b) { return a.m.nv > b.m.nv; };
+  for (auto& v : g32) std::stable_sort(v.begin(), v.end(), larger);
+  for (auto& v : g64) std::stable_sort(v.begin(), v.end(), larger);
+  mjhMulti* p = new mjhMulti{buckets};
+  int rc = MJH_OK;
+  for (size_t k = 0; k < p->buckets.size() && rc == MJH_OK; k++) {
+    Bucket& b = p->buckets[k];
+    rc = build_table(g32[k], 2, &b.t32);
+    if (rc == MJH_OK) rc = build_table(g32[k], 1, &b.t32_64);
+    if (rc == MJH_OK) rc = build_table(g64[k], 1, &b.t64);
+  }
   if (rc != MJH_OK) {
     free_tables(p);
     delete p;
@@ -905,20 +1071,23 @@ int mjh_multi_create(int ngroups, const mjhModel* models, const mjhData* datas, 
 }
 int mjh_multi_step(const mjhMulti* p, mjhStream stream) {
   if (!p) return fail(MJH_ERR_INVALID, "multi: null plan");
-  return multi_launches(*p, (hipStream_t)stream, true);
+  for (const Bucket& b : p->buckets) TRY(multi_launches(b, (hipStream_t)stream, true));
+  return MJH_OK;
 }
 int mjh_multi_step_static(int ngroups, const mjhModel* models, const mjhData* datas, mjhStream stream) {
   if (ngroups <= 0 || !models || !datas) return fail(MJH_ERR_INVALID, "multi_static: no groups");
   if (ngroups > MJH_MULTI_STATIC_MAX_GROUPS)
     return fail(MJH_ERR_CAPACITY, "multi_static: " + std::to_string(ngroups) + " groups; at most " +
                                       std::to_string(MJH_MULTI_STATIC_MAX_GROUPS) + " fit the library's table");
-  mjhMulti plan;
+  Bucket plan;
   int n32 = 0, n64 = 0;
   for (int i = 0; i < ngroups; i++) {
     GroupDesc gd{};
-    bool reg = false;
-    TRY(plan_group(i, models[i], datas[i], &plan, &gd, &reg));
-    (reg ? n32 : n64)++;
+    Choice c;
+    GroupLds l;
+    TRY(plan_group(i, models[i], datas[i], nullptr, &gd, &c, &l));
+    fold(&plan, l);
+    (c.reg ? n32 : n64)++;
   }
   TRY(plan_capacity(plan));
   GroupDesc* table = nullptr;
@@ -935,10 +1104,11 @@ int mjh_multi_step_static(int ngroups, const mjhModel* models, const mjhData* da
   int i32 = 0, i64 = 0;
   for (int i = 0; i < ngroups; i++) {
     GroupDesc gd{};
-    bool reg = false;
-    TRY(plan_group(i, models[i], datas[i], &plan, &gd, &reg));
+    Choice c;
+    GroupLds l;
+    TRY(plan_group(i, models[i], datas[i], nullptr, &gd, &c, &l));
     const int nw = datas[i].nworld;
-    if (reg) {
+    if (c.reg) {
       gd.first_block = (int)b32;
       hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(64), 0, s, gd, plan.t32.dev + i32, 0);
       gd.first_block = (int)b32_64;

@@ -27,7 +27,11 @@ MJH_DEV int level_end(const mjhModel& m, int l) {
 // enough for com_pos), crb (composite inertias, motion axes, dof parents
 // and the mass matrix), com_vel / rne (velocities, accelerations, forces)
 __host__ __device__ inline int lds_kinematics_floats(int nbody) { return 16 * nbody; }
-__host__ __device__ inline int lds_crb_floats(int nbody, int nv) { return 10 * nbody + 7 * nv + nv * nv; }
+// 32-lane teams build the mass matrix in LDS; 64-lane teams write it to
+// qMd directly (global: no nv^2 of LDS)
+__host__ __device__ inline int lds_crb_floats(int nbody, int nv, bool lds_matrix = true) {
+  return 10 * nbody + 7 * nv + (lds_matrix ? nv * nv : 0);
+}
 __host__ __device__ inline int lds_velocity_floats(int nbody) { return 12 * nbody; }
 
 // ---------------------------------------------------------------------------
@@ -274,11 +278,16 @@ MJH_DEV void factor_sparse(const mjhModel& m, const float* S, float* L, float* D
   sync();
 }
 
-// lds: lds_crb_floats; holds crb, cdof, dof_parentid and the mass matrix,
-// which is written to qMd row by row at the end (and, for sparse models, to
-// qM in MuJoCo's sparse layout)
+// lds: lds_crb_floats; holds crb, cdof, dof_parentid and, on 32-lane
+// teams, the mass matrix, which is written to qMd row by row at the end (and,
+// for sparse models, to qM in MuJoCo's sparse layout).  64-lane teams write
+// the entries straight into qMd: the tree blocks are cleared first (entries
+// of dofs in different trees are zero for good and never written), then
+// every dof writes its row's ancestor entries and their mirror images -- the
+// same values, with no nv^2 of LDS.
 template <int TEAM>
 MJH_DEV void crb(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
+  constexpr bool in_lds = TEAM != 64;
   const int w = t.world;
   const int nbody = m.nbody, nv = m.nv;
   float* crbw = WPTR(d.crb, w, nbody * 10);
@@ -287,12 +296,19 @@ MJH_DEV void crb(const mjhModel& m, const mjhData& d, float* lds, const Team<TEA
   float* lcrb = lds;
   float* lcdof = lds + 10 * nbody;
   int* lpar = (int*)(lcdof + 6 * nv);
-  float* lqM = lcdof + 7 * nv;
+  float* const qMg = WPTR(d.qMd, w, nv * nv);
+  float* lqM = in_lds ? lcdof + 7 * nv : qMg;
   if (t.valid) {
     for (int i = t.lane; i < nbody * 10; i += TEAM) lcrb[i] = cinert[i];
     for (int i = t.lane; i < nv * 6; i += TEAM) lcdof[i] = cdofw[i];
     for (int i = t.lane; i < nv; i += TEAM) lpar[i] = m.dof_parentid[i];
-    for (int i = t.lane; i < nv * nv; i += TEAM) lqM[i] = 0.0f;
+    if (in_lds) {
+      for (int i = t.lane; i < nv * nv; i += TEAM) lqM[i] = 0.0f;
+    } else {
+      for_each_tree_block(m, [&](int dof, int n) {
+        for (int p = t.lane; p < n * n; p += TEAM) lqM[(dof + p / n) * nv + dof + p % n] = 0.0f;
+      });
+    }
   }
   sync();
   // composite inertia: children into parents, bodies attached to the world
@@ -319,9 +335,9 @@ MJH_DEV void crb(const mjhModel& m, const mjhData& d, float* lds, const Team<TEA
     }
   }
   sync();
-  float* qM = WPTR(d.qMd, w, nv * nv);
   if (t.valid) {
-    for (int i = t.lane; i < nv * nv; i += TEAM) qM[i] = lqM[i];
+    if (in_lds)
+      for (int i = t.lane; i < nv * nv; i += TEAM) qMg[i] = lqM[i];
     if (m.opt.is_sparse) dense_to_sparse(m, lqM, WPTR(d.qM, w, m.nM), t);
   }
   sync();

@@ -38,6 +38,7 @@ struct SolverWorld {
   float* qc;
   const float* qM;
   float* lds;
+  bool wide;  // H in efc_h instead of LDS
 
   MJH_DEV float* J(int r) const { return d.efc_J + (size_t)(radr + r) * nv; }
 
@@ -88,8 +89,8 @@ struct SolverWorld {
     if (m.opt.solver == mjhSOL_CG) {
       solve_blocks(m, d.qLDd + (size_t)w * nv * nv, Mgrad, grad, lds, t);
     } else {
-      float* H = lds;
-      float* x = lds + nv * nv;
+      float* H = wide ? d.efc_h + (size_t)w * nv * nv : lds;
+      float* x = wide ? lds : lds + nv * nv;
       const int ntri = nv * (nv + 1) / 2;
       for (int p = t.lane; p < ntri; p += TEAM) {
         const int i = m.dof_tri_row[p], j = m.dof_tri_col[p];
@@ -332,16 +333,17 @@ struct SolverWorld {
   }
 };
 
-// solver.py:872-1000 solve.  lds: nv*nv + nv floats.
+// solver.py:872-1000 solve.  lds: the largest tree block's n^2 + n floats
+// and nv*nv + nv (H and the right-hand side), or nv when wide (H in efc_h).
 template <int TEAM>
-MJH_DEV void solve(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
+MJH_DEV void solve(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t, bool wide = false) {
   if (!t.valid) return;
   const int w = t.world, nv = m.nv;
   SolverWorld<TEAM> s{m, d, t, w, nv, d.world_efc[2 * w], d.world_efc[2 * w + 1],
                       d.qacc + (size_t)w * nv, d.efc_Ma + (size_t)w * nv, d.efc_grad + (size_t)w * nv,
                       d.efc_Mgrad + (size_t)w * nv, d.efc_search + (size_t)w * nv, d.efc_mv + (size_t)w * nv,
                       d.qfrc_smooth + (size_t)w * nv, d.qacc_smooth + (size_t)w * nv,
-                      d.qfrc_constraint + (size_t)w * nv, d.qMd + (size_t)w * nv * nv, lds};
+                      d.qfrc_constraint + (size_t)w * nv, d.qMd + (size_t)w * nv * nv, lds, wide};
   const float* ws = d.qacc_warmstart + (size_t)w * nv;
   for (int i = t.lane; i < nv; i += TEAM) s.qacc[i] = ws[i];
   sync();

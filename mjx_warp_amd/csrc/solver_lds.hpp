@@ -22,11 +22,11 @@ namespace mjh {
 
 constexpr int kFlagDeferred = 1;
 
-// LDS floats per world: nv^2 (H) + 14 nv (vectors) + 8 rmax (rows)
-// + rj nv (staged Jacobian rows).  The mass matrix is read from global
-// memory, where it is read-only during the solve.
-__host__ __device__ inline int lds_solver_floats(int nv, int rmax, int rj) {
-  return nv * nv + 14 * nv + 8 * rmax + rj * nv;
+// LDS floats per world: nv^2 (H; none when wide: H in efc_h) + 14 nv
+// (vectors) + 8 rmax (rows) + rj nv (staged Jacobian rows).  The mass
+// matrix is read from global memory, where it is read-only during the solve.
+__host__ __device__ inline int lds_solver_floats(int nv, int rmax, int rj, bool wide = false) {
+  return (wide ? 0 : nv * nv) + 14 * nv + 8 * rmax + rj * nv;
 }
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -75,6 +75,14 @@ struct LdsSolver {
   float *H, *qacc, *Ma, *grad, *Mgrad, *search, *mv, *qs, *qas, *qc, *pg, *pmg, *x;
   float *Jaref, *jv, *D, *q0, *q1, *q2, *force, *act;
   const float* J;  // row-major [nr, nv], LDS or global
+  // Newton: the factor in H is current.  H = M + J' D(active) J depends on
+  // the iterate only through the active set, so while no row changes
+  // activity the previous factor is the one a rebuild would produce, bit
+  // for bit.  act_bits: bit s set when row lane + TEAM s was active at the
+  // last factorisation; a world with more than 32 rows per lane refactors
+  // every iteration.
+  bool fact_valid = false;
+  unsigned act_bits = 0;
 
   MJH_DEV void mulM(float* res, const float* vec) const {
     for_each_tree_block(m, [&](int dof, int n) {
@@ -126,7 +134,23 @@ struct LdsSolver {
       gd += g * g;
     }
     gd = team_sum<TEAM>(gd);
-    if (m.opt.solver == mjhSOL_NEWTON && TEAM == 64) {
+    bool refactor = true;
+    if (m.opt.solver == mjhSOL_NEWTON) {
+      unsigned bits = 0;
+      bool many = false;
+      for (int r = t.lane, k = 0; r < nr; r += TEAM, k++) {
+        if (k >= 32) many = true;
+        else if (act[r] != 0.0f) bits |= 1u << k;
+      }
+      const float changed = team_sum<TEAM>(bits != act_bits || many ? 1.0f : 0.0f);
+      refactor = !fact_valid || changed != 0.0f;
+      act_bits = bits;
+      fact_valid = true;
+    }
+    if (m.opt.solver == mjhSOL_NEWTON && !refactor) {
+      sync();
+      cho_solve_lds(H, nv, nv, x, t);
+    } else if (m.opt.solver == mjhSOL_NEWTON && TEAM == 64) {
       hessian_mfma(M, J, D, act, nr, nv, H, t.lane);
       sync();
       chol_lds(H, nv, nv, m, t);
@@ -319,7 +343,8 @@ struct LdsSolver {
 // Returns false (and leaves the world untouched) when the world's rows do
 // not fit the LDS row capacity.
 template <int TEAM>
-MJH_DEV bool solve_lds(const mjhModel& m, const mjhData& d, float* lds, int rmax, int rj, const Team<TEAM>& t) {
+MJH_DEV bool solve_lds(const mjhModel& m, const mjhData& d, float* lds, int rmax, int rj, bool wide,
+                       const Team<TEAM>& t) {
   if (!t.valid) return true;
   const int w = t.world, nv = m.nv;
   const int radr = d.world_efc[2 * w], nr = d.world_efc[2 * w + 1];
@@ -327,7 +352,13 @@ MJH_DEV bool solve_lds(const mjhModel& m, const mjhData& d, float* lds, int rmax
   LdsSolver<TEAM> s{m, d, t, w, nv, radr, nr};
   float* p = lds;
   s.M = d.qMd + (size_t)w * nv * nv;
-  s.H = p; p += nv * nv;
+  // wide: H in the world's efc_h (CG: its tree-block factor in qLDd)
+  if (wide) {
+    s.H = m.opt.solver == mjhSOL_CG ? d.qLDd + (size_t)w * nv * nv : d.efc_h + (size_t)w * nv * nv;
+  } else {
+    s.H = p;
+    p += nv * nv;
+  }
   s.qacc = p; p += nv;
   s.Ma = p; p += nv;
   s.grad = p; p += nv;
@@ -358,7 +389,7 @@ MJH_DEV bool solve_lds(const mjhModel& m, const mjhData& d, float* lds, int rmax
   }
   // stage the world's inputs
   const size_t o = (size_t)w * nv;
-  if (m.opt.solver == mjhSOL_CG) {
+  if (m.opt.solver == mjhSOL_CG && !wide) {
     const float* qLD = d.qLDd + o * nv;
     for (int k = t.lane; k < nv * nv; k += TEAM) s.H[k] = qLD[k];
   }

@@ -23,10 +23,13 @@ namespace mjh {
 
 constexpr int kTeam = 64;
 
-// launch-time tuning of the LDS solver: row capacity and staged J rows
+// launch-time tuning of the solvers: the LDS solver's row capacity and
+// staged J rows; wide: the Newton Hessians of the 64-lane solvers live in
+// efc_h (global) instead of LDS, for models whose nv^2 does not fit
 struct Tune {
   int rmax;
   int rj;
+  int wide;
 };
 
 #define STAGE(NAME, CALL)                                                                              \
@@ -66,7 +69,7 @@ STAGE(StQaccSmooth, if (t.valid) for (int i = t.lane; i < m.nv; i += T) d.qacc[t
 template <int T>
 MJH_DEV bool solve_team(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) {
   if constexpr (T == kRegTeam) return solve_reg<2>(m, d, l, tune.rj, t);
-  else return solve_lds(m, d, l, tune.rmax, tune.rj, t);
+  else return solve_lds(m, d, l, tune.rmax, tune.rj, tune.wide != 0, t);
 }
 // Worlds the 32-lane solve launches leave to the big-world launch: more
 // rows than two row sets hold, at most kRegRows.
@@ -111,9 +114,9 @@ struct StBigWorlds {
 template <bool integrate>
 struct StTail {
   template <int T>
-  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune&) const {
+  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
     if (!t.valid || !(d.world_flags[t.world] & kFlagDeferred)) return;
-    solve(m, d, l, t);
+    solve(m, d, l, t, tune.wide != 0);
     sync();
     if (integrate) {
       if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
@@ -167,7 +170,8 @@ struct StFusedStep {
   }
 };
 
-// The fused step in two launches, for 32-lane teams: the forward stages
+// The fused step in two launches (32-lane teams, and 64-lane teams through
+// the group kernels): the forward stages
 // that walk the body tree, collide and build constraint rows (up to
 // qfrc_smooth), then everything that factors or solves with the mass
 // matrix: factor_m, qacc_smooth, the solver and the integrator.  The second
@@ -299,12 +303,13 @@ using StTailStep = StTail<true>;
 #define MJH_KERNELS_BIG_STEP_G(X, G) G(32, StBigWorldsStep, 1)
 #define MJH_KERNELS_TAIL_STEP(X, G) X(64, StTailStep) G(64, StTailStep, 0)
 #define MJH_KERNELS_FUSED64(X, G) X(64, StFusedStep) G(64, StFusedStep, 0)
+#define MJH_KERNELS_SPLIT64(X, G) G(64, StFusedForward, 0) G(64, StFusedSolveIntegrate, 0)
 
 #define MJH_KERNEL_LISTS(X, G)                                                                          \
   MJH_KERNELS_TREE(X, G) MJH_KERNELS_COLLISION(X, G) MJH_KERNELS_INTEGRATE(X, G) MJH_KERNELS_SOLVE(X, G) \
   MJH_KERNELS_BIG_SOLVE(X, G) MJH_KERNELS_BIG_SOLVE_G(X, G) MJH_KERNELS_FUSED_FORWARD(X, G)             \
   MJH_KERNELS_FUSED_SOLVE(X, G) MJH_KERNELS_FUSED_SOLVE_G(X, G) MJH_KERNELS_BIG_STEP(X, G)              \
-  MJH_KERNELS_BIG_STEP_G(X, G) MJH_KERNELS_TAIL_STEP(X, G) MJH_KERNELS_FUSED64(X, G)
+  MJH_KERNELS_BIG_STEP_G(X, G) MJH_KERNELS_TAIL_STEP(X, G) MJH_KERNELS_FUSED64(X, G) MJH_KERNELS_SPLIT64(X, G)
 
 #define MJH_INSTANTIATE_STAGE(TEAM, S) template __global__ void k_stage<TEAM, S>(mjhModel, mjhData, int, Tune);
 #define MJH_INSTANTIATE_GROUPS(TEAM, S, TI) \

@@ -236,6 +236,8 @@ def put_model(mjm, device=None) -> types.Model:
   m.nact_group = len(tables["actuator_moment_tileadr"])
   m.max_level_width = int(tables["max_level_width"])
   m.nnxn = len(tables["nxn_pair"])
+  # dofs of the largest kinematic tree: the LDS a tree block's factor takes
+  m.tree_dof_max = int(np.max(tables["qLD_tilesize"])) if len(tables["qLD_tilesize"]) else 0
   m.actuator_affine_bias_gain = int(
     bool(np.any(np.asarray(mjm.actuator_biastype) == 1) or np.any(np.asarray(mjm.actuator_gaintype) == 1)))
 

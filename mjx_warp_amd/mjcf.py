@@ -967,6 +967,17 @@ def _fk_com(m: MjModel, qpos):
   return xpos, xquat, xipos, subtree_com, cdof
 
 
+def _chain_dofs(m: MjModel, b) -> np.ndarray:
+  """Dofs of body b and its ancestors, ascending: the columns where a point
+  on b has a nonzero Jacobian."""
+  out = []
+  c = b
+  while c != 0:
+    out.extend(range(m.body_dofadr[c], m.body_dofadr[c] + m.body_dofnum[c]))
+    c = m.body_parentid[c]
+  return np.array(sorted(out), dtype=np.int64)
+
+
 def _body_jacobian(m: MjModel, point, b, subtree_com, cdof):
   """Translational and rotational Jacobians (3 x nv) of a point on body b."""
   jp = np.zeros((3, m.nv))
@@ -974,33 +985,34 @@ def _body_jacobian(m: MjModel, point, b, subtree_com, cdof):
   if b == 0:
     return jp, jr
   off = point - subtree_com[m.body_rootid[b]]
-  for i in range(m.nv):
-    a = m.dof_bodyid[i]
-    c = b
-    inside = False
-    while c != 0:
-      if c == a:
-        inside = True
-        break
-      c = m.body_parentid[c]
-    if inside:
-      jr[:, i] = cdof[i, :3]
-      jp[:, i] = cdof[i, 3:] + np.cross(cdof[i, :3], off)
+  for i in _chain_dofs(m, b):
+    jr[:, i] = cdof[i, :3]
+    jp[:, i] = cdof[i, 3:] + np.cross(cdof[i, :3], off)
   return jp, jr
 
 
 def mass_matrix(m: MjModel, qpos) -> np.ndarray:
-  """Dense joint-space inertia (armature included) at qpos, float64."""
+  """Dense joint-space inertia (armature included) at qpos, float64; each
+  body adds to the rows / columns of its own and its ancestors' dofs."""
   _, xquat, xipos, subtree_com, cdof = _fk_com(m, qpos)
   M = np.diag(m.dof_armature.astype(np.float64)).copy()
   for b in range(1, m.nbody):
     if m.body_mass[b] <= 0 and not np.any(m.body_inertia[b]):
       continue
     jp, jr = _body_jacobian(m, xipos[b], b, subtree_com, cdof)
+    idx = _chain_dofs(m, b)
+    jp, jr = jp[:, idx], jr[:, idx]
     R = _q2m(_qmul(xquat[b], m.body_iquat[b]))
     Iw = R @ np.diag(m.body_inertia[b]) @ R.T
-    M += m.body_mass[b] * jp.T @ jp + jr.T @ Iw @ jr
+    M[np.ix_(idx, idx)] += m.body_mass[b] * jp.T @ jp + jr.T @ Iw @ jr
   return M
+
+
+def _tree_blocks(m: MjModel):
+  """[begin, end) dof ranges of the kinematic trees (dofs of a tree are
+  contiguous, roots have dof_parentid -1)."""
+  roots = [i for i in range(m.nv) if m.dof_parentid[i] == -1]
+  return [(r, roots[k + 1] if k + 1 < len(roots) else m.nv) for k, r in enumerate(roots)]
 
 
 def set_const(m: MjModel):
@@ -1012,7 +1024,10 @@ def set_const(m: MjModel):
     return
   M = mass_matrix(m, m.qpos0)
   m.stat.meaninertia = float(np.trace(M) / m.nv)
-  Minv = np.linalg.inv(M)
+  # M is block diagonal over the kinematic trees: invert block by block
+  Minv = np.zeros_like(M)
+  for a, e in _tree_blocks(m):
+    Minv[a:e, a:e] = np.linalg.inv(M[a:e, a:e])
   dinv = np.diag(Minv).copy()
   m.dof_invweight0 = np.zeros(m.nv)
   for j in range(m.njnt):
@@ -1030,8 +1045,9 @@ def set_const(m: MjModel):
     if m.body_weldid[b] == 0:
       continue
     jp, jr = _body_jacobian(m, xipos[b], b, subtree_com, cdof)
-    J = np.vstack([jp, jr])
-    A = J @ Minv @ J.T
+    idx = _chain_dofs(m, b)
+    J = np.vstack([jp, jr])[:, idx]
+    A = J @ Minv[np.ix_(idx, idx)] @ J.T
     m.body_invweight0[b, 0] = max(_MINVAL, np.trace(A[:3, :3]) / 3.0)
     m.body_invweight0[b, 1] = max(_MINVAL, np.trace(A[3:, 3:]) / 3.0)
 

@@ -21,6 +21,13 @@
                     free spheres of the given radii (density 1000) in a
                     1 m box (floor and four walls at x, y = +-0.5); one model
                     per (links, spheres) pair, worlds split over the pairs.
+                    The reduced workload draws 2-8 links and 1-4 spheres
+                    (nv <= 32); `heterogeneous_groups(max_links=32,
+                    max_spheres=64)` is the workload at its stated size
+                    (nv up to 416), with chains longer than 9 links folded
+                    (`chain_fold`) so that they start inside the box.  The
+                    chain-workload links (0.5 m) would not fit a 1 m box past
+                    two links, so these chains take 0.1 m links.
 """
 
 from __future__ import annotations
@@ -101,7 +108,8 @@ def box_world_xml(links: int, radii) -> str:
   hinges about y limited to +-1.57) hung from (0, 0, 0.95), and one free
   sphere per radius (density 1000), inside a 1 m box: a floor and four walls
   (planes at x, y = +-0.5 facing inward).  Newton, implicitfast,
-  dt 0.002."""
+  dt 0.002, dense mass matrix at every size (implicitfast takes no sparse
+  one)."""
   body = ""
   for i in reversed(range(links)):
     pos = "0 0 0.95" if i == 0 else "0 0 -0.1"
@@ -113,7 +121,8 @@ def box_world_xml(links: int, radii) -> str:
   return f"""
 <mujoco model="box{links}x{len(radii)}">
   <compiler angle="radian"/>
-  <option timestep="0.002" integrator="implicitfast" solver="Newton" iterations="20" ls_iterations="10"/>
+  <option timestep="0.002" integrator="implicitfast" solver="Newton" iterations="20" ls_iterations="10"
+          jacobian="dense"/>
   <worldbody>
     <geom name="floor" type="plane" size="0 0 0.05"/>
     <geom name="wall_px" type="plane" pos="0.5 0 0.5" zaxis="-1 0 0" size="0.5 0.5 0.05"/>
@@ -130,18 +139,42 @@ def box_world(links: int, radii) -> mjcf.MjModel:
   return mjcf.from_xml_string(box_world_xml(links, radii))
 
 
+def chain_fold(links: int) -> np.ndarray:
+  """Hinge angles that keep a chain of `links` 0.1 m links inside the box:
+  hanging straight up to 9 links; longer chains fold into a zig-zag in the
+  x-z plane (runs of four links 1.45 rad from the vertical, alternating
+  direction, joined by a vertical link; every hinge within its +-1.57
+  limit), which stays within 0.4 m of x = 0 and drops about 0.3 m per ten
+  links."""
+  if links <= 9:
+    return np.zeros(links)
+  phi, sign = [], 1.0
+  while len(phi) < links:
+    phi += [sign * 1.45] * 4 + [0.0]
+    sign = -sign
+  phi = np.array(phi[:links])
+  return np.diff(np.concatenate([[0.0], phi]))
+
+
 def box_world_states(mjm, links: int, radii, nworld: int, rng):
-  """(qpos, qvel) [nworld, nq] / [nworld, nv]: the chain hanging straight,
-  each sphere at rest at a uniform position inside the box (clear of the
-  walls, the floor, the chain's column |x|, |y| < 0.1 + r, and the other
-  spheres; rejection sampling), identity orientation."""
+  """(qpos, qvel) [nworld, nq] / [nworld, nv]: the chain in its chain_fold
+  pose, each sphere at rest at a uniform position inside the box (clear of
+  the walls, the floor, the other spheres and the chain -- the column
+  |x|, |y| < 0.1 + r of a straight chain, the slab |y| < 0.05 + r,
+  |x| < 0.45 + r of a folded one; rejection sampling), identity
+  orientation."""
   q = np.zeros((nworld, mjm.nq))
+  fold = chain_fold(links)
+  q[:, :links] = fold
+  folded = np.any(fold != 0.0)
   for w in range(nworld):
     placed = []
     for k, r in enumerate(radii):
       for _ in range(1000):
         p = rng.uniform([-0.5 + r, -0.5 + r, r], [0.5 - r, 0.5 - r, 0.9 - r])
-        if abs(p[0]) < 0.1 + r and abs(p[1]) < 0.1 + r:
+        if not folded and abs(p[0]) < 0.1 + r and abs(p[1]) < 0.1 + r:
+          continue
+        if folded and abs(p[1]) < 0.05 + r and abs(p[0]) < 0.45 + r:
           continue
         if all(np.linalg.norm(p - c) > r + rc for c, rc in placed):
           break
@@ -150,6 +183,25 @@ def box_world_states(mjm, links: int, radii, nworld: int, rng):
       q[w, a:a + 3] = p
       q[w, a + 3] = 1.0
   return q, np.zeros((nworld, mjm.nv))
+
+
+def _box_world_entry(args):
+  return box_world(*args)
+
+
+def box_worlds(specs, processes: int = 1) -> list:
+  """box_world(links, radii) for every (links, radii, ...) of specs, in a
+  pool of `processes` worker processes (spawned: they compile MJCF only)
+  when there are many."""
+  todo = [(links, radii) for links, radii, *_ in specs]
+  if processes <= 1 or len(todo) < 2 * processes:
+    return [box_world(*a) for a in todo]
+  import concurrent.futures
+  import multiprocessing
+
+  ctx = multiprocessing.get_context("spawn")
+  with concurrent.futures.ProcessPoolExecutor(processes, mp_context=ctx) as ex:
+    return list(ex.map(_box_world_entry, todo, chunksize=max(1, len(todo) // (8 * processes))))
 
 
 def heterogeneous_groups(nworld: int = 16384, seed: int = 2, max_links: int = 8, max_spheres: int = 4):

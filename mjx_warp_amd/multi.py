@@ -12,10 +12,21 @@ it issues one `mjh_step_fused` per group; the "static" mode runs the
 grouped launches through `mjh_multi_step_static`, which keeps the group
 table in a fixed device array of the library and allocates nothing per
 call (for callers that cannot allocate, e.g. under graph capture; at most
-64 groups, and calls on one GPU must not overlap).  `autotune=True` times the grouped and per-group modes
-on the groups' own state (restored afterwards) and keeps the faster one,
-remembered per group signature (each group's sizes and world count) for
-the rest of the process.
+64 groups, and calls on one GPU must not overlap).
+
+Launch choices per group (`tunes`, the grouped mode): lanes per world (32:
+two worlds per wavefront with the register solver, nv <= 32 only; 64: one
+world per wavefront with the LDS solver), the LDS solver's row capacity and
+the Jacobian rows staged in LDS -- (team, rmax, rj), 0 / 0 / -1 for the
+library's choice; `buckets=True` launches groups of different LDS
+occupancy classes separately, each launch sized for its own groups.  Every
+choice computes the same step.
+
+`autotune=True` picks them by timing on the groups' own state (restored
+after every trial): first each group alone over its candidate choices
+(`candidates`), then the whole batch grouped with and without buckets and
+per group; the fastest is kept, remembered per group signature (each
+group's sizes and world count) for the rest of the process.
 
 The plan holds the groups' device pointers: write Data fields in place
 (`d.qpos.copy_(...)`); a field replaced by a new tensor is picked up by
@@ -35,6 +46,18 @@ _TUNED: dict = {}
 _STATE = ("qpos", "qvel", "act", "ctrl", "qacc_warmstart")
 
 
+def candidates(m) -> list:
+  """The autotuner's (team, rmax, rj) choices for a model, the library's
+  own first."""
+  out = [(0, 0, -1)]
+  if int(m.nv) <= 32:
+    out += [(32, 0, 32), (32, 0, 64), (32, 0, 0)]
+  out += [(64, 0, 32), (64, 0, 0), (64, 0, 64)]
+  if int(m.nv) > 32:
+    out.append((64, 0, 128))
+  return out
+
+
 def _signature(groups) -> tuple:
   return tuple((int(m.nq), int(m.nv), int(m.nbody), int(m.ngeom), int(m.nu), int(d.nworld), int(d.njmax))
                for m, d in groups)
@@ -43,13 +66,17 @@ def _signature(groups) -> tuple:
 class MultiStep:
   """One fused step of every world of every (Model, Data) group per call."""
 
-  def __init__(self, groups, autotune: bool = False, mode: str = "grouped"):
+  def __init__(self, groups, autotune: bool = False, mode: str = "grouped", tunes=None, buckets: bool = False):
     if not groups:
       raise ValueError("MultiStep needs at least one (Model, Data) group")
     if mode not in ("grouped", "per_group", "static"):
       raise ValueError(f"unknown mode {mode!r}")
     self.groups = [(m, d) for m, d in groups]
+    if tunes is not None and len(tunes) != len(self.groups):
+      raise ValueError("tunes needs one (team, rmax, rj) per group")
     self.mode = mode
+    self.tunes = list(tunes) if tunes is not None else None
+    self.buckets = bool(buckets)
     self._plan = None
     self._structs = None
     self._build()
@@ -57,7 +84,8 @@ class MultiStep:
       key = _signature(self.groups)
       if key not in _TUNED:
         _TUNED[key] = self._tune()
-      self.mode = _TUNED[key]
+      self.mode, self.tunes, self.buckets = _TUNED[key]
+      self._build()
 
   def _build(self):
     lib = _native.lib()
@@ -71,10 +99,14 @@ class MultiStep:
     self._structs = ds
     if self.mode == "static":
       return  # no plan: the library's own table
+    tunes = None
+    if self.tunes is not None:
+      tunes = (_native.GroupTuneStruct * n)(*[_native.GroupTuneStruct(*map(int, t)) for t in self.tunes])
     plan = ctypes.c_void_p()
-    rc = lib.mjh_multi_create(n, M, D, ctypes.byref(plan))
+    rc = lib.mjh_multi_create_tuned(n, M, D, tunes, _native.MJH_MULTI_BUCKETS if self.buckets else 0,
+                                    ctypes.byref(plan))
     if rc != _native.MJH_OK:
-      _native._raise(rc, "mjh_multi_create")
+      _native._raise(rc, "mjh_multi_create_tuned")
     self._plan = plan
 
   def _stale(self) -> bool:
@@ -100,30 +132,57 @@ class MultiStep:
     for m, d in self.groups:
       d.time = d.time + m.opt.timestep
 
-  def _tune(self, steps: int = 5) -> str:
+  def _timed(self, steps: int) -> float:
+    import torch
+
+    self.step()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+      self.step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+  def _tune(self, steps: int = 5) -> tuple:
+    """(mode, tunes, buckets) of the fastest trial."""
     import torch
 
     saved = [({f: getattr(d, f).clone() for f in _STATE}, d.time) for _, d in self.groups]
 
-    def restore():
-      for (_, d), (fields, t) in zip(self.groups, saved):
+    def restore(idx):
+      for i in idx:
+        (_, d), (fields, t) = self.groups[i], saved[i]
         for f, v in fields.items():
           getattr(d, f).copy_(v)
         d.time = t
 
-    best, best_t = self.mode, float("inf")
-    for mode in ("grouped", "per_group"):
-      self.mode = mode
-      self.step()  # warm-up
-      torch.cuda.synchronize()
-      t0 = time.perf_counter()
-      for _ in range(steps):
-        self.step()
-      torch.cuda.synchronize()
-      dt = time.perf_counter() - t0
-      restore()
+    everything = range(len(self.groups))
+    # each group alone over its candidates; another choice replaces the
+    # library's only when it is at least 3% faster
+    tunes = []
+    for i, (m, d) in enumerate(self.groups):
+      best, best_t = (0, 0, -1), float("inf")
+      for cand in candidates(m):
+        try:
+          one = MultiStep([(m, d)], mode="grouped", tunes=[cand])
+        except (ValueError, NotImplementedError):
+          continue  # does not fit this device's LDS
+        dt = one._timed(steps)
+        one.close()
+        restore([i])
+        if dt < (best_t if cand == (0, 0, -1) or best_t == float("inf") else 0.97 * best_t):
+          best, best_t = cand, dt
+      tunes.append(best)
+    # the whole batch: grouped with and without buckets, per group
+    trials = [("grouped", tunes, False), ("grouped", tunes, True), ("per_group", None, False)]
+    best, best_t = trials[0], float("inf")
+    for mode, tn, bk in trials:
+      self.mode, self.tunes, self.buckets = mode, tn, bk
+      self._build()
+      dt = self._timed(steps)
+      restore(everything)
       if dt < best_t:
-        best, best_t = mode, dt
+        best, best_t = (mode, tn, bk), dt
     torch.cuda.synchronize()
     return best
 

@@ -106,7 +106,8 @@ def tangent_grad(mjm, qpos, g_raw):
 
 def _ctrl_sides(mjm, u, eps):
   """(up, down) masks [n, nu]: control may move by +eps / -eps in range."""
-  lim = np.asarray(mjm.actuator_ctrllimited, bool) & ~bool(int(mjm.opt.disableflags) & (1 << 7))
+  clamp_on = not (int(mjm.opt.disableflags) & (1 << 7))
+  lim = np.asarray(mjm.actuator_ctrllimited, bool) & clamp_on
   lo, hi = np.asarray(mjm.actuator_ctrlrange, np.float64).T
   up = ~lim | (u + eps <= hi)
   down = ~lim | (u - eps >= lo)

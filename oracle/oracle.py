@@ -145,6 +145,10 @@ def run(stage: str, m: HostModel, d: HostData, nthreads: int = 1):
     raise ValueError("model and data precision differ")
   suf = "f32" if m.real == np.float32 else "f64"
   ms, ds = m.struct(), d.struct()
+  if stage in ("step", "implicit") and ms.opt.is_sparse and ms.opt.integrator == 3:
+    # implicitfast works on the dense matrices only; the reference refuses
+    # a sparse model (forward.py:268)
+    raise NotImplementedError("implicitfast with a sparse mass matrix is not implemented")
   rc = getattr(lib(), f"oracle_{stage}_{suf}")(ctypes.byref(ms), ctypes.byref(ds), int(nthreads))
   if rc != 0:
     raise RuntimeError(f"oracle {stage} failed: {rc}")

@@ -93,9 +93,10 @@ def pairs_of(hd, world=0):
   return sorted(tuple(sorted((int(a), int(b)))) for a, b in p)
 
 
-def oracle_stepper(mjm, m_cpu, real=np.float64, nconmax_w=64, njmax_w=277):
+def oracle_stepper(mjm, m_cpu, real=np.float64, nconmax_w=64, njmax_w=277, nthreads=1):
   """step(qpos, qvel, act, ctrl, warm) -> (qpos', qvel', act', warm') over
-  [n, .] arrays, one oracle step of n worlds (host data cached per n)."""
+  [n, .] arrays, one oracle step of n worlds (host data cached per n) on
+  nthreads host threads."""
   hm = host_model_exact(mjm, m_cpu, real) if real == np.float64 else host_model(m_cpu, real)
   cache = {}
 
@@ -112,7 +113,7 @@ def oracle_stepper(mjm, m_cpu, real=np.float64, nconmax_w=64, njmax_w=277):
     if mjm.nu:
       hd["ctrl"][:] = u
     hd["qacc_warmstart"][:] = w
-    oracle.run("step", hm, hd)
+    oracle.run("step", hm, hd, nthreads=nthreads)
     act = hd["act"].copy() if mjm.na else np.zeros((n, 0))
     return (hd["qpos"].astype(np.float64), hd["qvel"].astype(np.float64), act.astype(np.float64),
             hd["qacc_warmstart"].astype(np.float64))

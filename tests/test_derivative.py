@@ -237,6 +237,32 @@ def test_rollout_gradient_smooth_matches_oracle_fd():
 
 
 @pytest.mark.gpu
+def test_rollout_gradient_smooth_8_worlds_50_steps():
+  """The differentiable-rollout check at its stated size: 8 worlds of the
+  rollout workload, 50 steps, contacts and joint limits disabled: the
+  device adjoint's gradients (FP32, eps 4e-3) with respect to all 1050
+  ctrl entries and qvel0 against central differences (eps 1e-6) of whole
+  rollouts of the FP64 oracle.  qvel0 holds 1e-3; the ctrl gradient is
+  held to 1e-2 (measured 4.4e-3): the FP32 finite-difference Jacobians
+  carry ~1e-3 relative error in their qvel rows per step (the same error
+  the float32 oracle's own Jacobians show against float64: the rounding of
+  M^-1 in FP32, not the differencing), and the adjoint sums 50 of them."""
+  mjm = _humanoid()
+  mjm.opt.disableflags |= int(mjw.DisableBit.CONTACT) | int(mjw.DisableBit.LIMIT)
+  n, T = 8, 50
+  q0, v0, ctrl = _rollout_case(mjm, n, T)
+  gu, gv0 = _gpu_rollout_grads(mjm, q0, v0, ctrl)
+  step = helpers.oracle_stepper(mjm, mjw.put_model(mjm, device="cpu"), nconmax_w=4, njmax_w=4, nthreads=16)
+  z0, w0 = np.zeros((n, 0)), np.zeros((n, mjm.nv))
+  fd_u, fd_v0 = od.loss_grad_fd(step, _loss, q0, v0, z0, ctrl, w0, 1e-6)
+  eu, ev = _rel(gu, fd_u), _rel(gv0, fd_v0)
+  worst = max(_rel(gu[:, w], fd_u[:, w]) for w in range(n))
+  print(f"smooth, 8 worlds x 50 steps: ctrl gradient rel. error {eu:.2e} (worst world {worst:.2e}), "
+        f"qvel0 {ev:.2e}")
+  assert eu < 1e-2 and ev < 1e-3, (eu, ev)
+
+
+@pytest.mark.gpu
 def test_rollout_gradient_in_contact_matches_oracle_adjoint():
   """Humanoids in contact with the floor: the device adjoint's gradients
   against the oracle's adjoint over FP64 Jacobians at the same eps (4e-3);

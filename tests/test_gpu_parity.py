@@ -582,6 +582,33 @@ def test_step_model_over_64k_of_lds(fused):
   _close_or_enveloped("qacc", helpers.data_numpy(d, "qacc"), hd["qacc"], env["qacc"], 2e-3)
 
 
+@pytest.mark.parametrize("count", [3, 8])
+def test_wide_hessian_equals_lds_hessian(count, monkeypatch):
+  """The 64-lane solvers with the Newton Hessian in efc_h (wide: forced by
+  MJH_SOLVER_WIDE for three humanoids, taken by itself at nv 216 where
+  nv^2 floats pass a workgroup's LDS) against the oracle, and for three
+  humanoids bit for bit against the Hessian in LDS."""
+  mjm, nworld, q, v, u = _n_humanoids(count)
+  runs = []
+  for wide in ((False, True) if count == 3 else (None,)):
+    if wide:
+      monkeypatch.setenv("MJH_SOLVER_WIDE", "1")
+    m, d, hm, hd = _gpu_and_oracle(mjm, nworld, q, v, u, nconmax_w=32 * count, njmax_w=128 * count)
+    mjw.step_fused(m, d)
+    torch.cuda.synchronize()
+    monkeypatch.delenv("MJH_SOLVER_WIDE", raising=False)
+    runs.append({f: helpers.data_numpy(d, f).copy() for f in ("qpos", "qvel", "qacc")})
+  if count == 3:
+    for f in runs[0]:
+      assert np.array_equal(runs[0][f], runs[1][f]), (f, np.abs(runs[0][f] - runs[1][f]).max())
+  oracle.run("step", hm, hd)
+  assert not np.any(helpers.data_numpy(d, "world_flags"))
+  env = oracle_envelope(mjm, nworld, None, 32 * count, 128 * count, stage="step", fields=["qpos", "qvel"],
+                        inputs=(q, v, u), k=4)
+  _close_or_enveloped("qpos", runs[-1]["qpos"], hd["qpos"], env["qpos"], 1e-4)
+  _close_or_enveloped("qvel", runs[-1]["qvel"], hd["qvel"], env["qvel"], 2e-3)
+
+
 def test_forward_and_step_n_humanoids():
   """The reference's n_humanoids.xml compiled here (three attached
   humanoids, nv 81, 64-lane teams, Newton 100 / 50 by the file's defaults),

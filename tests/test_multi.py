@@ -7,8 +7,11 @@ thrown at a wall stays inside the box under the oracle's step.
 GPU: one grouped step per stage over groups of different models (32-lane
 box worlds and a humanoid, a 64-lane 40-link chain) equals each group's own
 fused step bit for bit; a captured graph of grouped steps replays the eager
-result; the box-world group matches the oracle's step; the autotuner picks a
-mode and leaves the state as it found it.
+result; the box-world group matches the oracle's step; plans with caller
+choices (64-lane teams for the small models, no staged Jacobian rows,
+buckets by LDS class) match the oracle too, buckets with the library's
+choices change nothing, and invalid choices are refused; the autotuner
+picks a mode and per-group choices and leaves the state as it found it.
 """
 
 import numpy as np
@@ -153,6 +156,142 @@ def test_grouped_step_matches_oracle():
       assert np.max(np.abs(a - b)) <= tol * (1 + np.max(np.abs(b))), (i, f, np.max(np.abs(a - b)))
 
 
+def _oracle_check(specs, groups, host):
+  for i, ((hm, hd), (m, d)) in enumerate(zip(host, groups)):
+    oracle.run("step", hm, hd)
+    for f, tol in (("qpos", 1e-4), ("qvel", 2e-3)):
+      a, b = helpers.data_numpy(d, f).astype(np.float64), hd[f].astype(np.float64)
+      assert np.max(np.abs(a - b)) <= tol * (1 + np.max(np.abs(b))), (i, f, np.max(np.abs(a - b)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("choice", ["team64", "rj0", "rmax_small", "buckets"])
+def test_tuned_plan_matches_oracle(choice):
+  """A grouped step with caller choices, every group against the oracle:
+  every group on 64-lane teams (the LDS solver for the nv <= 32 models),
+  no Jacobian rows staged in LDS, an LDS row capacity of 8 (worlds past it
+  go to the global-memory tail), or the library's choices in buckets."""
+  specs, make = _groups(seed=4)
+  groups = make()
+  host = []
+  for (mjm, *_), (m, d) in zip(specs, groups):
+    mc = mjw.put_model(mjm, device="cpu")
+    host.append((helpers.host_model(mc, np.float32), helpers.host_data(m, d, np.float32)))
+  tunes = {"team64": [(64, 0, -1)] * len(groups), "rj0": [(0, 0, 0)] * len(groups),
+           "rmax_small": [(64, 8, -1)] * len(groups), "buckets": None}[choice]
+  ms = mjw.MultiStep(groups, tunes=tunes, buckets=choice in ("buckets", "team64"))
+  ms.step()
+  torch.cuda.synchronize()
+  assert not any(np.any(helpers.data_numpy(d, "world_flags")) for _, d in groups)
+  _oracle_check(specs, groups, host)
+  ms.close()
+
+
+@pytest.mark.gpu
+def test_buckets_with_library_choices_change_nothing():
+  _, make = _groups(seed=5)
+  ga, gb = make(), make()
+  a, b = mjw.MultiStep(ga), mjw.MultiStep(gb, buckets=True)
+  for _ in range(3):
+    a.step()
+    b.step()
+  torch.cuda.synchronize()
+  for i, ((_, da), (_, db)) in enumerate(zip(ga, gb)):
+    for f in _FIELDS:
+      x, y = helpers.data_numpy(da, f), helpers.data_numpy(db, f)
+      assert np.array_equal(x, y), (i, f, np.abs(x - y).max())
+
+
+@pytest.mark.gpu
+def test_invalid_choices_are_refused():
+  _, make = _groups(seed=6)
+  groups = make()
+  n = len(groups)
+  with pytest.raises(ValueError, match="team must be"):
+    mjw.MultiStep(groups, tunes=[(48, 0, -1)] * n)
+  # the humanoid (nv 27) may take 32 lanes, the 40-link chain may not
+  with pytest.raises(ValueError, match="nv <= 32"):
+    mjw.MultiStep(groups, tunes=[(32, 0, -1)] * n)
+  with pytest.raises(ValueError, match="one"):
+    mjw.MultiStep(groups, tunes=[(0, 0, -1)])
+
+
+def _oracle_runs(mjm, n, q, v, ncon_w, njmax_w, rng):
+  """The oracle's step of one group in float32, float64 and twice in float32
+  from hinge angles / qvel moved by 1e-6: (qvel, cost) per run."""
+  hinge = [mjm.jnt_qposadr[j] for j in range(mjm.njnt) if mjm.jnt_type[j] in (2, 3)]
+  out = []
+  for i, real in enumerate((np.float32, np.float64, np.float32, np.float32)):
+    mc = mjw.put_model(mjm, device="cpu")
+    dc = mjw.make_data(mjm, nworld=n, nconmax=ncon_w * n, njmax=njmax_w * n, device="cpu")
+    qq, vv = q.copy(), v.copy()
+    if i >= 2:
+      qq[:, hinge] += 1e-6 * rng.normal(size=(n, len(hinge)))
+      vv = vv * (1 + 1e-6 * rng.normal(size=vv.shape))
+    helpers.set_field(dc, "qpos", qq)
+    helpers.set_field(dc, "qvel", vv)
+    hm = helpers.host_model(mc, real) if real == np.float32 else helpers.host_model_exact(mjm, mc, real)
+    hd = helpers.host_data(mc, dc, real)
+    oracle.run("step", hm, hd, nthreads=8)
+    out.append(dict(qpos=hd["qpos"].astype(np.float64), qvel=hd["qvel"].astype(np.float64),
+                    cost=hd["efc_cost"].astype(np.float64), nefc=int(hd["nefc"][0])))
+  return out
+
+
+@pytest.mark.gpu
+def test_stated_size_heterogeneous_worlds_match_oracle():
+  """Worlds of the heterogeneous workload at its stated size: chains up to
+  32 links and up to 64 spheres (nv up to 416: the Newton Hessian of the
+  largest does not fit a workgroup's LDS and lives in efc_h), spheres
+  resting on the floor, stepped together in one grouped step and each
+  group against the oracle.  qpos to 1e-4; qvel to 2e-3 or within twice
+  the oracle's own spread (its float64 run and two float32 runs from inputs
+  moved by 1e-6) -- the 416-dof worlds' 20 Newton iterations over ~670
+  rows end in a flat cost valley where the float32 and float64 oracle
+  runs are 0.17 apart in qvel at costs equal to 3e-7 -- or, failing both,
+  with the cost within 2.5% of the oracle's (solver_test.py:127)."""
+  rng = np.random.default_rng(7)
+  specs, groups = [], []
+  for links, nsph, n in ((32, 64, 2), (20, 30, 2), (12, 9, 3), (5, 3, 3)):
+    radii = tuple(float(r) for r in rng.uniform(0.03, 0.1, nsph))
+    mjm = models.box_world(links, radii)
+    q, _ = models.box_world_states(mjm, links, radii, n, rng)
+    for k, r in enumerate(radii):
+      q[:, links + 7 * k + 2] = r - 0.002  # on the floor
+    v = rng.normal(0, 0.3, (n, mjm.nv))
+    ncon_w = 3 * (links + nsph)
+    njmax_w = 4 * ncon_w + mjm.njnt
+    m = mjw.put_model(mjm)
+    d = mjw.make_data(mjm, nworld=n, nconmax=ncon_w * n, njmax=njmax_w * n)
+    helpers.set_field(d, "qpos", q)
+    helpers.set_field(d, "qvel", v)
+    specs.append((mjm, n, q, v, ncon_w, njmax_w))
+    groups.append((m, d))
+  assert max(int(m.nv) for m, _ in groups) == 416
+  ms = mjw.MultiStep(groups)
+  ms.step()
+  torch.cuda.synchronize()
+  by_cost = 0
+  for i, ((mjm, n, q, v, ncon_w, njmax_w), (m, d)) in enumerate(zip(specs, groups)):
+    runs = _oracle_runs(mjm, n, q, v, ncon_w, njmax_w, rng)
+    ref = runs[0]
+    assert ref["nefc"] > 0, i
+    assert int(helpers.data_numpy(d, "nefc")[0]) == ref["nefc"], i
+    qp = helpers.data_numpy(d, "qpos").astype(np.float64)
+    assert np.max(np.abs(qp - ref["qpos"])) <= 1e-4 * (1 + np.max(np.abs(ref["qpos"]))), (i, "qpos")
+    qv = helpers.data_numpy(d, "qvel").astype(np.float64)
+    cost = helpers.data_numpy(d, "efc_cost").astype(np.float64)
+    err = np.abs(qv - ref["qvel"]).max(axis=1)
+    spread = np.max([np.abs(r["qvel"] - ref["qvel"]).max(axis=1) for r in runs[1:]], axis=0)
+    ok = (err <= 2e-3 * (1 + np.abs(ref["qvel"]).max())) | (err <= 2.0 * spread)
+    costs = np.stack([r["cost"] for r in runs])
+    held = ~ok & (cost <= 1.025 * costs.max(0) + 1e-6) & (cost >= costs.min(0) / 1.025 - 1e-6)
+    assert np.all(ok | held), (i, err, spread)
+    by_cost += int(held.sum())
+  print(f"stated-size worlds: {by_cost} held by the cost criterion")
+  ms.close()
+
+
 @pytest.mark.gpu
 def test_autotune_picks_a_mode_and_restores_state():
   _, make = _groups(seed=3)
@@ -160,6 +299,10 @@ def test_autotune_picks_a_mode_and_restores_state():
   before = [{f: helpers.data_numpy(d, f).copy() for f in ("qpos", "qvel", "qacc_warmstart")} for _, d in groups]
   ms = mjw.MultiStep(groups, autotune=True)
   assert ms.mode in ("grouped", "per_group")
+  if ms.mode == "grouped":
+    assert len(ms.tunes) == len(groups)
+    for (m, _), t in zip(groups, ms.tunes):
+      assert t in mjw.multi.candidates(m), t
   torch.cuda.synchronize()
   for (_, d), b in zip(groups, before):
     for f, v in b.items():
