@@ -271,9 +271,7 @@ Tune tune_for(const mjhModel& m, bool reg) {
     const int fwd = lds_forward(m, false);
     if (fwd > base) base = fwd;
     if (base + 8 * kLdsRowsMin <= cap) {
-      // wide worlds are few per CU anyway and a row past the capacity sends
-      // the world to the slow tail: they take rows up to half the LDS
-      const int per_cu = t.wide ? 2 : cap / (base + 8 * kLdsRowsMin);
+      const int per_cu = cap / (base + 8 * kLdsRowsMin);
       const int rows = (cap / per_cu - lds_solver_floats(m.nv, 0, t.rj, t.wide)) / 8;
       t.rmax = rows < kLdsRowsMin ? kLdsRowsMin : rows > kLdsRowsMax ? kLdsRowsMax : rows;
     }
@@ -283,9 +281,23 @@ Tune tune_for(const mjhModel& m, bool reg) {
   if (t.rmax < 0) t.rmax = 0;
   if (t.rj > t.rmax) t.rj = t.rmax;
   if (t.rj < 0) t.rj = 0;
+  // on the 64-lane path the tail launch (the few worlds over rmax) takes the
+  // LDS solver with kLdsRowsMax rows when that fits one workgroup; 32-lane
+  // models reach the tail only past the big-world launch's 256 rows, rarely
+  // enough to leave it the global-memory solver and its small LDS
+  t.tail_rows = !reg && lds_solver_floats(m.nv, kLdsRowsMax, 0, t.wide != 0) <= (int)(lds_cap() / sizeof(float))
+                    ? kLdsRowsMax
+                    : 0;
   return t;
 }
 Tune tune_for(const mjhModel& m) { return tune_for(m, reg_path(m)); }
+// the tail launch: the global-memory solver's LDS and the LDS solver's
+// with the tail's row capacity
+int lds_tail(const mjhModel& m, const Tune& t) {
+  const int a = lds_tail(m, t.wide != 0);
+  const int b = t.tail_rows > 0 ? lds_solver_floats(m.nv, t.tail_rows, 0, t.wide != 0) : 0;
+  return a > b ? a : b;
+}
 int lds_solve(const mjhModel& m, const Tune& t) {
   const int a = lds_solver_floats(m.nv, t.rmax, t.rj, t.wide != 0), b = lds_linalg(m);
   return a > b ? a : b;
@@ -516,7 +528,7 @@ int do_solve(const mjhModel& m, const mjhData& d, hipStream_t s) {
   } else {
     TRY((launch_team<StSolveTeam, kTeam>(m, d, s, lds_solve(m), "solve")));
   }
-  return launch_team<StTail<false>, kTeam>(m, d, s, lds_tail(m, tune_for(m).wide != 0), "solve_tail");
+  return launch_team<StTail<false>, kTeam>(m, d, s, lds_tail(m, tune_for(m)), "solve_tail");
 }
 
 int do_fwd_position(const mjhModel& m, const mjhData& d, hipStream_t s) {
@@ -578,7 +590,7 @@ int do_step_fused(const mjhModel& m, const mjhData& d, hipStream_t s) {
   } else {
     TRY((launch_team<StFusedStep, kTeam>(m, d, s, lds_step(m), "step_fused")));
   }
-  return launch_team<StTail<true>, kTeam>(m, d, s, lds_tail(m, tune_for(m).wide != 0), "step_tail");
+  return launch_team<StTail<true>, kTeam>(m, d, s, lds_tail(m, tune_for(m)), "step_tail");
 }
 
 // one workgroup per group
@@ -706,7 +718,7 @@ int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
   GroupDesc* dev = reinterpret_cast<GroupDesc*>(d.step_desc);
   const bool reg = reg_path(m);
   const int l_fwd = lds_forward_reg(m), l_solve = lds_solve_reg(m), l_big = lds_tail_reg(m), l_step = lds_step(m),
-            l_tail = lds_tail(m, g.tune[0].wide != 0), l_fwd64 = lds_forward(m, false),
+            l_tail = lds_tail(m, g.tune[0]), l_fwd64 = lds_forward(m, false),
             l_solve64 = lds_solve(m, g.tune[0]);
   const size_t need = reg ? (size_t)max_of(max_of(2 * l_fwd, 2 * l_solve), max_of(2 * l_big, l_tail))
                           : (size_t)max_of(split64() ? max_of(l_fwd64, l_solve64) : l_step, l_tail);
@@ -771,10 +783,10 @@ GroupLds group_lds(const mjhModel& m, const Choice& c) {
     l.fwd = reg_stride(lds_forward(m, true));
     l.solve = lds_solve_reg(m, c.tune[0]);
     l.big = lds_tail_reg(m, c.tune[1]);
-    l.tail32 = lds_tail(m, false);
+    l.tail32 = lds_tail(m, c.tune[0]);
   } else {
     l.step = lds_step(m, c.tune[0]);
-    l.tail64 = lds_tail(m, c.tune[0].wide != 0);
+    l.tail64 = lds_tail(m, c.tune[0]);
     l.fwd64 = lds_forward(m, false);
     l.solve64 = lds_solve(m, c.tune[0]);
   }

@@ -25,11 +25,14 @@ constexpr int kTeam = 64;
 
 // launch-time tuning of the solvers: the LDS solver's row capacity and
 // staged J rows; wide: the Newton Hessians of the 64-lane solvers live in
-// efc_h (global) instead of LDS, for models whose nv^2 does not fit
+// efc_h (global) instead of LDS, for models whose nv^2 does not fit;
+// tail_rows: the row capacity of the LDS solver in the tail launch (0: the
+// tail has only the global-memory solver)
 struct Tune {
   int rmax;
   int rj;
   int wide;
+  int tail_rows;
 };
 
 #define STAGE(NAME, CALL)                                                                              \
@@ -109,14 +112,18 @@ struct StBigWorlds {
   }
 };
 
-// tail: flagged worlds get the global-memory solver (and, when `integrate`
-// is set, the integrator), then the flag is cleared
+// tail: flagged worlds get the LDS solver with a large row capacity or,
+// past it, the global-memory solver (and, when `integrate` is set, the
+// integrator), then the flag is cleared
 template <bool integrate>
 struct StTail {
   template <int T>
   MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
     if (!t.valid || !(d.world_flags[t.world] & kFlagDeferred)) return;
-    solve(m, d, l, t, tune.wide != 0);
+    // the LDS solver with the tail's row capacity, else the global one
+    bool solved = false;
+    if constexpr (T == kTeam) solved = tune.tail_rows > 0 && solve_lds(m, d, l, tune.tail_rows, 0, tune.wide != 0, t);
+    if (!solved) solve(m, d, l, t, tune.wide != 0);
     sync();
     if (integrate) {
       if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);

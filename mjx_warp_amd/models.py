@@ -220,3 +220,61 @@ def heterogeneous_groups(nworld: int = 16384, seed: int = 2, max_links: int = 8,
       if n:
         out.append((links, tuple(float(r) for r in rng.uniform(0.03, 0.1, s)), n))
   return out
+
+
+def contact_heavy_xml(nbody: int = 256, seed: int = 4) -> str:
+  """The contact-heavy workload's model: `nbody` free bodies, alternately
+  spheres (radius U(0.05, 0.1)) and capsules (radius 0.05, half-length
+  U(0.05, 0.15)), density 1000, in a 2 m box of 5 planes (floor and four
+  walls at x, y = +-1 facing inward); sizes drawn once (numpy
+  default_rng(seed)).  Newton (50 iterations, tol 1e-8), implicitfast,
+  dt 0.002, dense mass matrix.  261 geoms: sweep-and-prune broadphase."""
+  rng = np.random.default_rng(seed)
+  bodies = []
+  for i in range(nbody):
+    if i % 2 == 0:
+      geom = f'<geom type="sphere" size="{rng.uniform(0.05, 0.1):.6g}" density="1000"/>'
+    else:
+      geom = f'<geom type="capsule" size="0.05 {rng.uniform(0.05, 0.15):.6g}" density="1000"/>'
+    bodies.append(f'<body name="b{i}" pos="0 0 {1 + i}"><freejoint/>{geom}</body>')
+  return f"""
+<mujoco model="contact_heavy{nbody}">
+  <option timestep="0.002" integrator="implicitfast" solver="Newton" iterations="50" tolerance="1e-8"
+          jacobian="dense"/>
+  <worldbody>
+    <geom name="floor" type="plane" size="0 0 0.05"/>
+    <geom name="wall_px" type="plane" pos="1 0 1" zaxis="-1 0 0" size="1 1 0.05"/>
+    <geom name="wall_nx" type="plane" pos="-1 0 1" zaxis="1 0 0" size="1 1 0.05"/>
+    <geom name="wall_py" type="plane" pos="0 1 1" zaxis="0 -1 0" size="1 1 0.05"/>
+    <geom name="wall_ny" type="plane" pos="0 -1 1" zaxis="0 1 0" size="1 1 0.05"/>
+    {"".join(bodies)}
+  </worldbody>
+</mujoco>"""
+
+
+def contact_heavy(nbody: int = 256, seed: int = 4) -> mjcf.MjModel:
+  return mjcf.from_xml_string(contact_heavy_xml(nbody, seed))
+
+
+def contact_heavy_states(mjm, nworld: int, rng, height: float = 4.0):
+  """qpos [nworld, nq]: every body at a uniform position inside the box
+  (its bounding sphere clear of the walls, the floor and the bodies placed
+  before it; rejection sampling up to `height`) with a uniform random
+  orientation -- the bodies are dropped from rest."""
+  nb = mjm.nbody - 1
+  q = np.zeros((nworld, mjm.nq))
+  rb = np.asarray(mjm.geom_rbound)[5:5 + nb]
+  for w in range(nworld):
+    placed = []
+    for k in range(nb):
+      r = rb[k]
+      for _ in range(2000):
+        p = rng.uniform([-1 + r, -1 + r, r], [1 - r, 1 - r, height - r])
+        if all(np.linalg.norm(p - c) > r + rc for c, rc in placed):
+          break
+      placed.append((p, r))
+      quat = rng.normal(size=4)
+      q[w, 7 * k:7 * k + 3] = p
+      q[w, 7 * k + 3:7 * k + 7] = quat / np.linalg.norm(quat)
+  return q
+

@@ -492,6 +492,58 @@ def test_sap_matches_oracle_random_scenes():
            helpers.gather_world_rows(hd[f], ocon), 2e-4)
 
 
+def test_contact_heavy_collision_matches_oracle():
+  """The contact-heavy workload's scene (256 free spheres and capsules in a
+  2 m box, 261 geoms: sweep-and-prune), bodies dropped into a pile 0.8 m
+  high so that many touch: broadphase pairs and contacts per world against
+  the oracle.  A contact normal is the direction between the two closest
+  points of the geoms' centre segments, so it moves with rounding when
+  those points nearly coincide or the segments are nearly parallel (a
+  capsule pair 0.1 m deep has them 4e-4 apart: a few FP32 ulps of position
+  turn the normal by 2.7e-3; the float32 and float64 oracle frames differ by
+  up to 1.3e-3 here): distances and positions are held to 2e-4, frames on
+  98% of the contacts to 4e-4 or three times the oracle's own spread, on
+  all of them to 1e-2."""
+  mjm = models.contact_heavy()
+  nworld = 3
+  q = models.contact_heavy_states(mjm, nworld, np.random.default_rng(5), height=0.8)
+  m, d, hm, hd = _gpu_and_oracle(mjm, nworld, q, nconmax_w=2048, njmax_w=4)
+  mjw.kinematics(m, d)
+  mjw.collision(m, d)
+  torch.cuda.synchronize()
+  oracle.run("kinematics", hm, hd, nthreads=3)
+  oracle.run("collision", hm, hd, nthreads=3)
+  mc = mjw.put_model(mjm, device="cpu")
+  dc = mjw.make_data(mjm, nworld=nworld, nconmax=2048 * nworld, njmax=4 * nworld, device="cpu")
+  helpers.set_field(dc, "qpos", q)
+  hd64 = helpers.host_data(mc, dc, np.float64)
+  hm64 = helpers.host_model_exact(mjm, mc, np.float64)
+  oracle.run("kinematics", hm64, hd64, nthreads=3)
+  oracle.run("collision", hm64, hd64, nthreads=3)
+  gp = helpers.world_blocks(helpers.data_numpy(d, "world_pair"), nworld)
+  op = helpers.world_blocks(hd["world_pair"], nworld)
+  assert [n for _, n in gp] == [n for _, n in op]
+  assert np.array_equal(helpers.gather_world_rows(helpers.data_numpy(d, "collision_pair"), gp),
+                        helpers.gather_world_rows(hd["collision_pair"], op))
+  gcon = helpers.world_blocks(helpers.data_numpy(d, "world_con"), nworld)
+  ocon = helpers.world_blocks(hd["world_con"], nworld)
+  assert [n for _, n in gcon] == [n for _, n in ocon]
+  assert min(n for _, n in ocon) > 50, ocon
+  assert [n for _, n in helpers.world_blocks(hd64["world_con"], nworld)] == [n for _, n in ocon]
+  for f in CONTACT_FIELDS:
+    got = helpers.gather_world_rows(helpers.data_numpy(d, f), gcon)
+    ref = helpers.gather_world_rows(hd[f], ocon)
+    if f != "contact_frame":
+      _close(f, got, ref, 2e-4)
+      continue
+    r64 = helpers.gather_world_rows(hd64[f], helpers.world_blocks(hd64["world_con"], nworld))
+    err = np.abs(got.astype(np.float64) - ref).reshape(len(ref), -1).max(1)
+    spread = np.abs(r64 - ref).reshape(len(ref), -1).max(1)
+    tight = (err <= 4e-4) | (err <= 3 * spread)
+    print(f"contact frames: {np.mean(tight):.4f} within 4e-4 or the oracle's spread, max error {err.max():.2e}")
+    assert np.mean(tight) >= 0.98 and err.max() <= 1e-2, (np.mean(tight), err.max())
+
+
 def _world_match(a, b, tol):
   a = np.asarray(a, np.float64)
   b = np.asarray(b, np.float64)
