@@ -325,8 +325,9 @@ int lds_step(const mjhModel& m, const Tune& t) {
   return a > b ? a : b;
 }
 int lds_step(const mjhModel& m) { return lds_step(m, tune_for(m, false)); }
-// The 64-lane fused step as two launches (forward; solve and integrator),
-// each with its own LDS, or as one launch (MJH_FUSED64=one).
+// The 64-lane fused step as three launches (forward with the mass-matrix
+// factor; solver; integrator), each with its own LDS, or as one launch
+// (MJH_FUSED64=one).
 bool split64() {
   static const bool one = [] {
     const char* e = std::getenv("MJH_FUSED64");
@@ -718,7 +719,7 @@ int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
   GroupDesc* dev = reinterpret_cast<GroupDesc*>(d.step_desc);
   const bool reg = reg_path(m);
   const int l_fwd = lds_forward_reg(m), l_solve = lds_solve_reg(m), l_big = lds_tail_reg(m), l_step = lds_step(m),
-            l_tail = lds_tail(m, g.tune[0]), l_fwd64 = lds_forward(m, false),
+            l_tail = lds_tail(m, g.tune[0]), l_fwd64 = max_of(lds_forward(m, false), lds_linalg(m)),
             l_solve64 = lds_solve(m, g.tune[0]);
   const size_t need = reg ? (size_t)max_of(max_of(2 * l_fwd, 2 * l_solve), max_of(2 * l_big, l_tail))
                           : (size_t)max_of(split64() ? max_of(l_fwd64, l_solve64) : l_step, l_tail);
@@ -740,8 +741,9 @@ int do_step_fused_desc(const mjhModel& m, const mjhData& d, hipStream_t s) {
           return launch_groups<kRegTeam, StFusedSolveIntegrate, 0>(t32, l_solve, q, "step_fused_solve");
         }));
   } else if (split64()) {
-    TRY((launch_groups<kTeam, StFusedForward, 0>(t64, l_fwd64, s, "step_fused_forward")));
-    TRY((launch_groups<kTeam, StFusedSolveIntegrate, 0>(t64, l_solve64, s, "step_fused_solve")));
+    TRY((launch_groups<kTeam, StForwardFactor, 0>(t64, l_fwd64, s, "step_fused_forward")));
+    TRY((launch_groups<kTeam, StSolveStep, 0>(t64, l_solve64, s, "step_fused_solve")));
+    TRY((launch_groups<kTeam, StIntegrateSolved, 0>(t64, lds_linalg(m), s, "step_fused_integrate")));
   } else {
     TRY((launch_groups<kTeam, StFusedStep, 0>(t64, l_step, s, "step_fused")));
   }
@@ -787,7 +789,7 @@ GroupLds group_lds(const mjhModel& m, const Choice& c) {
   } else {
     l.step = lds_step(m, c.tune[0]);
     l.tail64 = lds_tail(m, c.tune[0]);
-    l.fwd64 = lds_forward(m, false);
+    l.fwd64 = max_of(lds_forward(m, false), lds_linalg(m));
     l.solve64 = lds_solve(m, c.tune[0]);
   }
   return l;
@@ -864,8 +866,9 @@ int multi_launches(const Bucket& p, hipStream_t s, bool use_side) {
   }
   if (p.t64.ngroups) {
     if (split64()) {
-      TRY((launch_groups<kTeam, StFusedForward, 0>(p.t64, p.lds_fwd64, s, "multi: forward")));
-      TRY((launch_groups<kTeam, StFusedSolveIntegrate, 0>(p.t64, p.lds_solve64, s, "multi: solve")));
+      TRY((launch_groups<kTeam, StForwardFactor, 0>(p.t64, p.lds_fwd64, s, "multi: forward")));
+      TRY((launch_groups<kTeam, StSolveStep, 0>(p.t64, p.lds_solve64, s, "multi: solve")));
+      TRY((launch_groups<kTeam, StIntegrateSolved, 0>(p.t64, p.lds_tail64, s, "multi: integrate")));
     } else {
       TRY((launch_groups<kTeam, StFusedStep, 0>(p.t64, p.lds_step, s, "multi: step")));
     }

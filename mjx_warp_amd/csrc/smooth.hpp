@@ -344,11 +344,13 @@ MJH_DEV void crb(const mjhModel& m, const mjhData& d, float* lds, const Team<TEA
 }
 
 // ---------------------------------------------------------------------------
-// Dense Cholesky of an n x n block (row stride ld) held in LDS: A = L L',
-// L overwrites the lower triangle and the strict upper triangle is zeroed,
-// matching wp.tile_cholesky as used by smooth.py:361-368.
+// Dense Cholesky of an n x n block (row stride ld) held in LDS (or, for a
+// wide Newton Hessian, in global memory): A = L L', L overwrites the lower
+// triangle and the strict upper triangle is zeroed, matching
+// wp.tile_cholesky as used by smooth.py:361-368.
 template <int TEAM>
 MJH_DEV void chol_lds(float* A, int n, int ld, const mjhModel& m, const Team<TEAM>& t) {
+  (void)m;
   for (int k = 0; k < n; k++) {
     const float dkk = sqrtf(A[k * ld + k]);
     const float inv = 1.0f / dkk;
@@ -356,14 +358,27 @@ MJH_DEV void chol_lds(float* A, int n, int ld, const mjhModel& m, const Team<TEA
     for (int i = k + 1 + t.lane; i < n; i += TEAM) A[i * ld + k] *= inv;
     if (t.lane == 0) A[k * ld + k] = dkk;
     sync();
-    // trailing update of the lower triangle of rows/cols k+1..n-1; the
-    // lower-triangle index tables enumerate (r, c) with c <= r
-    const int mrem = n - k - 1;
-    const int npair = mrem * (mrem + 1) / 2;
-    for (int p = t.lane; p < npair; p += TEAM) {
-      const int r = m.dof_tri_row[p] + k + 1;
-      const int c = m.dof_tri_col[p] + k + 1;
-      A[r * ld + c] -= A[r * ld + k] * A[c * ld + k];
+    // trailing update of the lower triangle of rows/cols k+1..n-1: lane
+    // per column c, down the rows r >= c, the scaled column k read as a
+    // broadcast (same entry for every lane) and the lanes' entries of a row
+    // adjacent (coalesced / conflict-free); each entry takes the same single
+    // update per k as in any order of the pairs
+    for (int c = k + 1 + t.lane; c < n; c += TEAM) {
+      const float lck = A[c * ld + k];
+      int r = c;
+      // eight rows' loads issued before their stores: the compiler cannot
+      // tell that column c's stores miss column k's loads
+      for (; r + 8 <= n; r += 8) {
+        float a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          a[u] = A[(r + u) * ld + k];
+          b[u] = A[(r + u) * ld + c];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) A[(r + u) * ld + c] = b[u] - a[u] * lck;
+      }
+      for (; r < n; r++) A[r * ld + c] -= A[r * ld + k] * lck;
     }
     sync();
   }
@@ -374,9 +389,55 @@ MJH_DEV void chol_lds(float* A, int n, int ld, const mjhModel& m, const Team<TEA
   sync();
 }
 
+// cho_solve_lds on one 64-lane team: the rows in blocks of 64, one per
+// lane.  A block first takes the terms of the entries already solved (lane
+// i sums over its row of L, in the same order as the column sweep below
+// would), then solves its diagonal block column by column, each solved
+// entry passed to the other lanes with readlane instead of through memory
+// and a barrier: one barrier per block instead of two per entry.  Every
+// entry gets the same operations in the same order as the column sweep.
+MJH_DEV void cho_solve_wave(const float* L, int n, int ld, float* x, int lane) {
+  // forward: y = inv(L) x
+  for (int b0 = 0; b0 < n; b0 += 64) {
+    const int i = b0 + lane, nb = n - b0 < 64 ? n - b0 : 64;
+    const bool own = i < n;
+    float y = own ? x[i] : 0.0f;
+    if (own)
+      for (int j = 0; j < b0; j++) y -= L[i * ld + j] * x[j];
+    for (int k = 0; k < nb; k++) {
+      const float yk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(y / L[(b0 + k) * ld + b0 + k]), k));
+      if (lane == k) y = yk;
+      else if (lane > k && own) y -= L[i * ld + b0 + k] * yk;
+    }
+    sync();
+    if (own) x[i] = y;
+    sync();
+  }
+  // backward: x = inv(L') y
+  for (int b0 = ((n - 1) >> 6) << 6; b0 >= 0; b0 -= 64) {
+    const int i = b0 + lane, nb = n - b0 < 64 ? n - b0 : 64;
+    const bool own = i < n;
+    float v = own ? x[i] : 0.0f;
+    if (own)
+      for (int j = n - 1; j >= b0 + 64; j--) v -= L[j * ld + i] * x[j];
+    for (int k = nb - 1; k >= 0; k--) {
+      const float xk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v / L[(b0 + k) * ld + b0 + k]), k));
+      if (lane == k) v = xk;
+      else if (lane < k && own) v -= L[(b0 + k) * ld + i] * xk;
+    }
+    sync();
+    if (own) x[i] = v;
+    sync();
+  }
+}
+
 // x = inv(L L') x for an n-vector x in LDS, L lower (row stride ld) in LDS.
 template <int TEAM>
 MJH_DEV void cho_solve_lds(const float* L, int n, int ld, float* x, const Team<TEAM>& t) {
+  if constexpr (TEAM == 64) {
+    cho_solve_wave(L, n, ld, x, t.lane);
+    return;
+  }
   for (int k = 0; k < n; k++) {
     const float xk = x[k] / L[k * ld + k];
     sync();

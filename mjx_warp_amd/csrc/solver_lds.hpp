@@ -47,15 +47,23 @@ MJH_DEV void hessian_mfma(const float* M, const float* J, const float* D, const 
     for (int tj = 0; tj <= ti; tj++) {
       floatx16 acc = {};
       const int ci = ti * 32 + c, cj = tj * 32 + c;
-      for (int k = 0; k < nr; k += 2) {
-        const int r = k + half;
-        float a = 0.0f, b = 0.0f;
-        if (r < nr) {
-          const float dr = D[r] * act[r];
-          if (ci < nv) a = dr * J[r * nv + ci];
-          if (cj < nv) b = J[r * nv + cj];
+      // the operands of eight row pairs are loaded before their matrix
+      // instructions, so that their loads are in flight together
+      for (int k0 = 0; k0 < nr; k0 += 16) {
+        float a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int r = k0 + 2 * u + half;
+          a[u] = 0.0f;
+          b[u] = 0.0f;
+          if (r < nr) {
+            const float dr = D[r] * act[r];
+            if (ci < nv) a[u] = dr * J[r * nv + ci];
+            if (cj < nv) b[u] = J[r * nv + cj];
+          }
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], acc, 0, 0, 0);
       }
 #pragma unroll
       for (int v = 0; v < 16; v++) {

@@ -220,6 +220,45 @@ struct StFusedSolveIntegrate {
   }
 };
 
+// The fused step on 64-lane teams in three launches: the forward stages
+// with the mass-matrix factor and qacc_smooth, the solver, the integrator.
+// Each launch takes the LDS its own stages need; the solver's LDS is what
+// limits its occupancy, and the stages around it keep theirs (one launch
+// for the whole step ran them at the solver's: the three attached
+// humanoids stepped 20% slower than per-stage launches).  One world per
+// workgroup, so the per-world early returns are uniform per workgroup.
+struct StForwardFactor {
+  template <int T>
+  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
+    StFusedForward()(m, d, l, t, tune);
+    sync();
+    factor_m(m, d, l, t);
+    sync();
+    fwd_acceleration_solve(m, d, l, t);
+  }
+};
+struct StSolveStep {
+  template <int T>
+  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune& tune) const {
+    if (d.njmax == 0) {
+      if (t.valid)
+        for (int i = t.lane; i < m.nv; i += T) d.qacc[t.world * m.nv + i] = d.qacc_smooth[t.world * m.nv + i];
+      return;
+    }
+    if (!solve_team(m, d, l, t, tune) && t.lane == 0) d.world_flags[t.world] |= kFlagDeferred;
+  }
+};
+// worlds the solve launch deferred are integrated by the tail launch
+struct StIntegrateSolved {
+  template <int T>
+  MJH_DEV void operator()(const mjhModel& m, const mjhData& d, float* l, const Team<T>& t, const Tune&) const {
+    static_assert(T == 64, "one world per workgroup");
+    if (t.valid && (d.world_flags[t.world] & kFlagDeferred)) return;
+    if (m.opt.integrator == mjhINT_EULER) euler(m, d, l, t);
+    else implicit(m, d, l, t);
+  }
+};
+
 template <int TEAM, class S>
 __global__ void __launch_bounds__(64) k_stage(mjhModel m, mjhData d, int lds_per_world, Tune tune) {
   static_assert(64 % TEAM == 0, "teams tile the wavefront");
@@ -310,7 +349,7 @@ using StTailStep = StTail<true>;
 #define MJH_KERNELS_BIG_STEP_G(X, G) G(32, StBigWorldsStep, 1)
 #define MJH_KERNELS_TAIL_STEP(X, G) X(64, StTailStep) G(64, StTailStep, 0)
 #define MJH_KERNELS_FUSED64(X, G) X(64, StFusedStep) G(64, StFusedStep, 0)
-#define MJH_KERNELS_SPLIT64(X, G) G(64, StFusedForward, 0) G(64, StFusedSolveIntegrate, 0)
+#define MJH_KERNELS_SPLIT64(X, G) G(64, StForwardFactor, 0) G(64, StSolveStep, 0) G(64, StIntegrateSolved, 0)
 
 #define MJH_KERNEL_LISTS(X, G)                                                                          \
   MJH_KERNELS_TREE(X, G) MJH_KERNELS_COLLISION(X, G) MJH_KERNELS_INTEGRATE(X, G) MJH_KERNELS_SOLVE(X, G) \
