@@ -228,10 +228,10 @@ int lds_linalg(const mjhModel& m) {
   return m.opt.is_sparse && m.nM > a ? m.nM : a;
 }
 // the global-memory solver of the tail launches: per-block linear algebra
-// and its Newton Hessian with right-hand side (nv^2 + nv), or only the
-// right-hand side when wide (H in efc_h)
+// and its Newton Hessian with right-hand side (nv^2 + nv), or when wide (H
+// in efc_h) the right-hand side and the blocked Cholesky's panel buffer
 int lds_tail(const mjhModel& m, bool wide) {
-  const int a = lds_linalg(m), h = wide ? m.nv : m.nv * m.nv + m.nv;
+  const int a = lds_linalg(m), h = wide ? m.nv + chol_blocked_floats(m.nv) : m.nv * m.nv + m.nv;
   return a > h ? a : h;
 }
 int lds_sap(const mjhModel& m) {

@@ -389,6 +389,85 @@ MJH_DEV void chol_lds(float* A, int n, int ld, const mjhModel& m, const Team<TEA
   sync();
 }
 
+// chol_lds for a matrix in global memory (a wide Newton Hessian), in
+// panels of kCholPanel columns: the panel's diagonal block is factored in
+// LDS by chol_lds, the rows below it are solved against that block (lane
+// per row, the row's panel entries in registers) and kept in LDS, and the
+// trailing lower triangle takes their products lane per column.  Every
+// entry gets the same operations in the same order as in chol_lds (so the
+// factor is the same bit for bit) with 1/kCholPanel of the global-memory
+// traffic.  lds: chol_blocked_floats(n).
+constexpr int kCholPanel = 16;
+__host__ __device__ inline int chol_blocked_floats(int n) { return kCholPanel * kCholPanel + kCholPanel * n; }
+
+template <int TEAM>
+MJH_DEV void chol_blocked(float* A, int n, int ld, float* lds, const mjhModel& m, const Team<TEAM>& t) {
+  constexpr int B = kCholPanel;
+  float* D = lds;          // the diagonal block, row stride nb
+  float* P = lds + B * B;  // the panel rows below it, row stride B
+  for (int p = 0; p < n; p += B) {
+    const int nb = n - p < B ? n - p : B;
+    for (int e = t.lane; e < nb * nb; e += TEAM) D[e] = A[(p + e / nb) * ld + p + e % nb];
+    sync();
+    chol_lds(D, nb, nb, m, t);
+    for (int e = t.lane; e < nb * nb; e += TEAM) A[(p + e / nb) * ld + p + e % nb] = D[e];
+    for (int i = p + nb + t.lane; i < n; i += TEAM) {
+      float x[B];
+#pragma unroll
+      for (int k = 0; k < B; k++) x[k] = k < nb ? A[i * ld + p + k] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < B; k++) {
+        if (k < nb) {
+#pragma unroll
+          for (int j = 0; j < k; j++) x[k] -= x[j] * D[k * nb + j];
+          x[k] *= 1.0f / D[k * nb + k];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < B; k++)
+        if (k < nb) {
+          A[i * ld + p + k] = x[k];
+          P[(i - p - nb) * B + k] = x[k];
+        }
+    }
+    sync();
+    for (int c = p + nb + t.lane; c < n; c += TEAM) {
+      float pc[B];
+#pragma unroll
+      for (int k = 0; k < B; k++) pc[k] = k < nb ? P[(c - p - nb) * B + k] : 0.0f;
+      int i = c;
+      for (; i + 4 <= n; i += 4) {
+        float a[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) a[u] = A[(i + u) * ld + c];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const float* pi = P + (i + u - p - nb) * B;
+#pragma unroll
+          for (int k = 0; k < B; k++)
+            if (k < nb) a[u] -= pi[k] * pc[k];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) A[(i + u) * ld + c] = a[u];
+      }
+      for (; i < n; i++) {
+        float a = A[i * ld + c];
+        const float* pi = P + (i - p - nb) * B;
+#pragma unroll
+        for (int k = 0; k < B; k++)
+          if (k < nb) a -= pi[k] * pc[k];
+        A[i * ld + c] = a;
+      }
+    }
+    sync();
+  }
+  for (int e = t.lane; e < n * n; e += TEAM) {
+    const int r = e / n, c = e % n;
+    if (c > r) A[r * ld + c] = 0.0f;
+  }
+  sync();
+}
+
 // cho_solve_lds on one 64-lane team: the rows in blocks of 64, one per
 // lane.  A block first takes the terms of the entries already solved (lane
 // i sums over its row of L, in the same order as the column sweep below

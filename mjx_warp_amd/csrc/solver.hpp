@@ -91,6 +91,7 @@ struct SolverWorld {
     } else {
       float* H = wide ? d.efc_h + (size_t)w * nv * nv : lds;
       float* x = wide ? lds : lds + nv * nv;
+      float* pbuf = wide ? lds + nv : nullptr;
       const int ntri = nv * (nv + 1) / 2;
       for (int p = t.lane; p < ntri; p += TEAM) {
         const int i = m.dof_tri_row[p], j = m.dof_tri_col[p];
@@ -106,7 +107,8 @@ struct SolverWorld {
       }
       for (int i = t.lane; i < nv; i += TEAM) x[i] = grad[i];
       sync();
-      chol_lds(H, nv, nv, m, t);
+      if (pbuf) chol_blocked(H, nv, nv, pbuf, m, t);
+      else chol_lds(H, nv, nv, m, t);
       cho_solve_lds(H, nv, nv, x, t);
       for (int i = t.lane; i < nv; i += TEAM) Mgrad[i] = x[i];
     }
@@ -334,7 +336,8 @@ struct SolverWorld {
 };
 
 // solver.py:872-1000 solve.  lds: the largest tree block's n^2 + n floats
-// and nv*nv + nv (H and the right-hand side), or nv when wide (H in efc_h).
+// and nv*nv + nv (H and the right-hand side), or when wide (H in efc_h) nv
+// and chol_blocked's panel buffer.
 template <int TEAM>
 MJH_DEV void solve(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t, bool wide = false) {
   if (!t.valid) return;

@@ -22,11 +22,12 @@ namespace mjh {
 
 constexpr int kFlagDeferred = 1;
 
-// LDS floats per world: nv^2 (H; none when wide: H in efc_h) + 14 nv
-// (vectors) + 8 rmax (rows) + rj nv (staged Jacobian rows).  The mass
+// LDS floats per world: nv^2 (H; when wide H is in efc_h and this is the
+// blocked Cholesky's panel buffer) + 14 nv (vectors) + 8 rmax (rows) + rj nv
+// (staged Jacobian rows).  The mass
 // matrix is read from global memory, where it is read-only during the solve.
 __host__ __device__ inline int lds_solver_floats(int nv, int rmax, int rj, bool wide = false) {
-  return (wide ? 0 : nv * nv) + 14 * nv + 8 * rmax + rj * nv;
+  return (wide ? chol_blocked_floats(nv) : nv * nv) + 14 * nv + 8 * rmax + rj * nv;
 }
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -83,6 +84,7 @@ struct LdsSolver {
   float *H, *qacc, *Ma, *grad, *Mgrad, *search, *mv, *qs, *qas, *qc, *pg, *pmg, *x;
   float *Jaref, *jv, *D, *q0, *q1, *q2, *force, *act;
   const float* J;  // row-major [nr, nv], LDS or global
+  float* pbuf = nullptr;  // wide: chol_blocked's panel buffer
   // Newton: the factor in H is current.  H = M + J' D(active) J depends on
   // the iterate only through the active set, so while no row changes
   // activity the previous factor is the one a rebuild would produce, bit
@@ -161,7 +163,8 @@ struct LdsSolver {
     } else if (m.opt.solver == mjhSOL_NEWTON && TEAM == 64) {
       hessian_mfma(M, J, D, act, nr, nv, H, t.lane);
       sync();
-      chol_lds(H, nv, nv, m, t);
+      if (pbuf) chol_blocked(H, nv, nv, pbuf, m, t);
+      else chol_lds(H, nv, nv, m, t);
       cho_solve_lds(H, nv, nv, x, t);
     } else if (m.opt.solver == mjhSOL_NEWTON) {
       const int ntri = nv * (nv + 1) / 2;
@@ -363,6 +366,8 @@ MJH_DEV bool solve_lds(const mjhModel& m, const mjhData& d, float* lds, int rmax
   // wide: H in the world's efc_h (CG: its tree-block factor in qLDd)
   if (wide) {
     s.H = m.opt.solver == mjhSOL_CG ? d.qLDd + (size_t)w * nv * nv : d.efc_h + (size_t)w * nv * nv;
+    s.pbuf = p;
+    p += chol_blocked_floats(nv);
   } else {
     s.H = p;
     p += nv * nv;
