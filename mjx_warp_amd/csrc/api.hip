@@ -250,23 +250,28 @@ constexpr int kLdsRowsMin = 256, kLdsRowsMax = 1024;
 // 64-lane teams.  On the 64-lane path no rows are staged (the LDS solver
 // stages a world's rows only when all of them fit, and the LDS that takes
 // costs occupancy: the three attached humanoids step 1.25x faster without
-// it); the Newton Hessian moves to efc_h (wide) when with it in LDS next to
-// 256 rows a CU would hold a single workgroup (five attached humanoids,
-// nv 135, step 2.2x faster wide); and the row capacity fills what
+// it); the Newton Hessian moves to efc_h (wide, factored there in LDS
+// panels) when with it in LDS next to 256 rows a CU would hold fewer than
+// five workgroups (three attached humanoids, nv 81: 1.03e6 world-steps/s
+// wide, 0.89e6 in LDS; a 40-link chain, nv 40: 2.3e6 wide, 2.75e6 in LDS);
+// and the row capacity fills what
 // the solver's LDS leaves before a CU would hold one workgroup fewer (a CU
 // holds lds_cap() bytes: 160 KiB on gfx950): rows cost 8 floats each, and a
 // world with more rows than the capacity is solved by the slower
 // global-memory tail.  MJH_SOLVER_RMAX / MJH_SOLVER_RJ override the
-// defaults, MJH_SOLVER_WIDE=1 takes the Hessian to efc_h for any 64-lane
-// model.
+// defaults, MJH_SOLVER_WIDE=1 / 0 puts the Hessian of any 64-lane model in
+// efc_h / in LDS (where it fits).
 Tune tune_for(const mjhModel& m, bool reg) {
   Tune t{64, reg ? 32 : 0, 0};
   if (!reg) {
     t.rmax = kLdsRowsMin;
     const int cap = (int)(lds_cap() / sizeof(float));
-    const char* force_wide = std::getenv("MJH_SOLVER_WIDE");
-    t.wide = lds_solver_floats(m.nv, kLdsRowsMin, t.rj) > cap / 2 || lds_tail(m, false) > cap ||
-             (force_wide && std::strcmp(force_wide, "1") == 0);
+    const bool fits = lds_solver_floats(m.nv, kLdsRowsMin, t.rj) <= cap && lds_tail(m, false) <= cap;
+    t.wide = !fits || lds_solver_floats(m.nv, kLdsRowsMin, t.rj) > cap / 5;
+    if (const char* e = std::getenv("MJH_SOLVER_WIDE")) {
+      if (std::strcmp(e, "1") == 0) t.wide = 1;
+      if (std::strcmp(e, "0") == 0 && fits) t.wide = 0;
+    }
     int base = lds_solver_floats(m.nv, 0, t.rj, t.wide);
     const int fwd = lds_forward(m, false);
     if (fwd > base) base = fwd;

@@ -636,15 +636,16 @@ def test_step_model_over_64k_of_lds(fused):
 
 @pytest.mark.parametrize("count", [3, 8])
 def test_wide_hessian_equals_lds_hessian(count, monkeypatch):
-  """The 64-lane solvers with the Newton Hessian in efc_h (wide: forced by
-  MJH_SOLVER_WIDE for three humanoids, taken by itself at nv 216 where
-  nv^2 floats pass a workgroup's LDS) against the oracle, and for three
-  humanoids bit for bit against the Hessian in LDS."""
+  """The 64-lane solvers with the Newton Hessian in efc_h (wide, factored
+  in LDS panels: the library's choice for three humanoids, and at nv 216
+  where nv^2 floats pass a workgroup's LDS) against the oracle, and for
+  three humanoids bit for bit against the Hessian in LDS (MJH_SOLVER_WIDE
+  forces either)."""
   mjm, nworld, q, v, u = _n_humanoids(count)
   runs = []
   for wide in ((False, True) if count == 3 else (None,)):
-    if wide:
-      monkeypatch.setenv("MJH_SOLVER_WIDE", "1")
+    if wide is not None:
+      monkeypatch.setenv("MJH_SOLVER_WIDE", "1" if wide else "0")
     m, d, hm, hd = _gpu_and_oracle(mjm, nworld, q, v, u, nconmax_w=32 * count, njmax_w=128 * count)
     mjw.step_fused(m, d)
     torch.cuda.synchronize()
