@@ -353,14 +353,11 @@ def time_heterogeneous(args, nworld: int = 16384, steps: int = 50, graph_steps: 
   return out
 
 
-def time_heterogeneous_stated(args, nworld: int = 16384, steps: int = 100):
-  """World-steps/s of the heterogeneous workload at its stated size: 16,384
-  worlds, each a hinge chain of U{2..32} 0.1 m capsule links (folded inside
-  the box) plus U{1..64} free spheres (radius U(0.03, 0.1), density 1000) in
-  a 1 m box of 5 planes, Newton -- one model per (links, spheres) pair
-  present, nv 8 to 416 -- advanced together by the grouped step (with and
-  without buckets by LDS class, the faster kept), `steps` steps captured as
-  one HIP graph and replayed (the value), and eagerly."""
+def stated_groups(nworld: int, processes: int = 16):
+  """(Model, Data) groups of the heterogeneous workload at its stated shape
+  (chains of U{2..32} links, U{1..64} spheres in a box of 5 planes), one per
+  (links, spheres) pair present among `nworld` worlds, with their initial
+  poses; returns the groups and the seconds spent compiling the models."""
   import numpy as np
   import torch
 
@@ -369,7 +366,7 @@ def time_heterogeneous_stated(args, nworld: int = 16384, steps: int = 100):
 
   t0 = time.perf_counter()
   spec = models.heterogeneous_groups(nworld, max_links=32, max_spheres=64)
-  mjms = models.box_worlds(spec, processes=min(16, os.cpu_count() or 1))
+  mjms = models.box_worlds(spec, processes=min(processes, os.cpu_count() or 1))
   t_compile = time.perf_counter() - t0
   rng = np.random.default_rng(2)
   groups = []
@@ -381,6 +378,23 @@ def time_heterogeneous_stated(args, nworld: int = 16384, steps: int = 100):
     d.qpos.copy_(torch.tensor(q, dtype=torch.float32))
     groups.append((m, d))
   torch.cuda.synchronize()
+  return groups, t_compile
+
+
+def time_heterogeneous_stated(args, nworld: int = 16384, steps: int = 100):
+  """World-steps/s of the heterogeneous workload at its stated size: 16,384
+  worlds, each a hinge chain of U{2..32} 0.1 m capsule links (folded inside
+  the box) plus U{1..64} free spheres (radius U(0.03, 0.1), density 1000) in
+  a 1 m box of 5 planes, Newton -- one model per (links, spheres) pair
+  present, nv 8 to 416 -- advanced together by the grouped step (with and
+  without buckets by LDS class, the faster kept), `steps` steps captured as
+  one HIP graph and replayed (the value), and eagerly."""
+  import torch
+
+  import mjx_warp_amd as mjw
+
+  t0 = time.perf_counter()
+  groups, t_compile = stated_groups(nworld)
   t_setup = time.perf_counter() - t0
   log(f"stated-size heterogeneous: {len(groups)} models, setup {t_setup:.1f} s (compile {t_compile:.1f} s)")
   state = [{f: getattr(d, f).clone() for f in ("qpos", "qvel", "qacc_warmstart")} for _, d in groups]
@@ -494,6 +508,64 @@ def copy_probe(mib: int, reps: int = 5):
   log(f"copy probe: {reps} copies of {mib} MiB ({4 * n} bytes read and written each)")
 
 
+def hetero_stated_probe(nworld: int, warm: int = 40, steps: int = 5):
+  """A short run of the stated-shape heterogeneous workload at `nworld`
+  worlds for a kernel trace: the grouped step with buckets by LDS class,
+  `warm` untimed steps, then `steps` timed; prints one line and returns."""
+  import torch
+
+  import mjx_warp_amd as mjw
+
+  groups, _ = stated_groups(nworld, processes=1)
+  log(f"stated-shape heterogeneous probe: {len(groups)} models, {nworld} worlds")
+  ms = mjw.MultiStep(groups, buckets=True)
+  for k in range(warm):
+    ms.step()
+    if k % 10 == 0:
+      torch.cuda.synchronize()
+      log(f"step {k}")
+  torch.cuda.synchronize()
+  t = time.perf_counter()
+  for _ in range(steps):
+    ms.step()
+  torch.cuda.synchronize()
+  dt = time.perf_counter() - t
+  rows = sum(int(d.nefc[0]) for _, d in groups) / nworld
+  log(f"{len(groups)} models, {nworld} worlds: {nworld * steps / dt:.4g} world-steps/s, {rows:.1f} rows per world")
+  ms.close()
+
+
+def collision_probe(nworld: int, calls: int = 20):
+  """The contact-heavy scene (128 spheres, 128 capsules, 5 planes: 261
+  geoms, so collision takes sweep-and-prune) at `nworld` worlds: times
+  sap_broadphase and collision over `calls` calls each; prints and returns."""
+  import numpy as np
+  import torch
+
+  import mjx_warp_amd as mjw
+  from mjx_warp_amd import models
+
+  if nworld % 64:
+    raise SystemExit("--collision-probe takes a multiple of 64 worlds (64 distinct drops, tiled)")
+  mjm = models.contact_heavy()
+  q = np.tile(models.contact_heavy_states(mjm, 64, np.random.default_rng(0)), (nworld // 64, 1))
+  m = mjw.put_model(mjm)
+  d = mjw.make_data(mjm, nworld=nworld, nconmax=2048 * nworld, njmax=4 * nworld)
+  d.qpos.copy_(torch.tensor(q, dtype=torch.float32))
+  mjw.kinematics(m, d)
+  for _ in range(3):
+    mjw.collision(m, d)
+  torch.cuda.synchronize()
+  for name, fn in (("sap_broadphase", mjw.sap_broadphase), ("collision", mjw.collision)):
+    t = time.perf_counter()
+    for _ in range(calls):
+      fn(m, d)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / calls
+    log(f"{name}: {nworld} worlds, {dt * 1e3:.3f} ms per call, {nworld / dt:.4g} worlds/s, "
+        f"pairs {int(d.ncollision[0])} contacts {int(d.ncon[0])}")
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -514,9 +586,20 @@ def main():
   ap.add_argument("--graph", action="store_true", help="replay a HIP graph of one captured step")
   ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                   help="only run a plain copy of MIB MiB (counter calibration) and exit")
+  ap.add_argument("--hetero-stated-probe", type=int, default=0, metavar="NWORLD",
+                  help="only run a short grouped rollout of the stated-shape heterogeneous workload "
+                       "at NWORLD worlds (kernel traces) and exit")
+  ap.add_argument("--collision-probe", type=int, default=0, metavar="NWORLD",
+                  help="only time the contact-heavy scene's collision at NWORLD worlds and exit")
   args = ap.parse_args()
   if args.copy_probe:
     copy_probe(args.copy_probe)
+    return
+  if args.hetero_stated_probe:
+    hetero_stated_probe(args.hetero_stated_probe)
+    return
+  if args.collision_probe:
+    collision_probe(args.collision_probe)
     return
   if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
     print("bench.py measures one GPU: every box here has one MI355X and the step needs no "
