@@ -549,6 +549,25 @@ int mjh_transition_fd(const mjhModel* m, const mjhData* d, const mjhData* fd, fl
 int mjh_adjoint_step(const mjhModel* m, const mjhData* d, const float* A, const float* B, const float* g_qpos,
                      const float* g_qvel, const float* g_act, const float* lam_next, float* lam, float* grad_ctrl,
                      mjhStream stream);
+/* The analytic vector-Jacobian product of one step (no counterpart in the
+ * reference, which has no gradients; MuJoCo's transition derivative
+ * conventions): d holds the step just taken from the state (qpos
+ * [nworld, nq], qvel [nworld, nv]) -- its forward quantities at that state
+ * and qvel' in d->qvel, as mjh_step_fused leaves them --, lam_next
+ * [nworld, nx] is the adjoint of the successor state.  Writes
+ *   lam = (dx'/dx)' lam_next (+ gadd [nworld, nx] if not null),
+ *   grad_ctrl = (dx'/du)' lam_next   [nworld, nu],
+ * by reverse mode through the integrator, the smooth dynamics (rne, the
+ * mass matrix, springs, dampers, actuators) and implicit differentiation of
+ * the converged constraint optimum with its active set held (one solve
+ * with M + dt diag(damping) and one with M + J_A' D_A J_A per world);
+ * contact points and frames follow the narrowphase's plane and sphere /
+ * capsule pair geometry (other pairs: held frames).  Euler or
+ * implicitfast, na = 0, no affine actuators, free / hinge / slide joints
+ * (the caller checks joint types); a world with more active rows than fit
+ * the LDS of one workgroup gets NaN gradients. */
+int mjh_adjoint_vjp(const mjhModel* m, const mjhData* d, const float* qpos, const float* qvel, const float* lam_next,
+                    const float* gadd, float* lam, float* grad_ctrl, mjhStream stream);
 /* lam [nworld, nx] as gradients with respect to d's raw state: g_qpos
  * [nworld, nq] (a rotation's tangent gradient g becomes 2 q (0, g),
  * orthogonal to q), g_qvel [nworld, nv], g_act [nworld, na]. */

@@ -218,6 +218,7 @@ def lib():
   L.mjh_transition_fd.argtypes = [P, P, P, ctypes.c_float, ctypes.c_int, P, P, P]
   L.mjh_adjoint_step.argtypes = [P, P, P, P, P, P, P, P, P, P, P]
   L.mjh_adjoint_state_grad.argtypes = [P, P, P, P, P, P, P]
+  L.mjh_adjoint_vjp.argtypes = [P, P, P, P, P, P, P, P, P]
   return L
 
 

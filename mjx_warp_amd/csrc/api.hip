@@ -20,6 +20,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include "adjoint.hpp"
 #include "stages.hpp"
 
 namespace mjh {
@@ -212,6 +213,13 @@ __global__ void __launch_bounds__(64) k_adjoint_step(mjhModel m, mjhData d, cons
                                                      const float* lam_next, float* lam, float* grad_ctrl) {
   Team<kTeam> t(d.nworld);
   adjoint_step(m, d, A, B, g_qpos, g_qvel, g_act, lam_next, lam, grad_ctrl, t);
+}
+__global__ void __launch_bounds__(64) k_adjoint_vjp(mjhModel m, mjhData d, const float* qpos_t, const float* qvel_t,
+                                                    const float* lam_next, const float* gadd, float* lam,
+                                                    float* grad_ctrl, int rcap) {
+  extern __shared__ float smem[];
+  Team<kTeam> t(d.nworld);
+  adjoint_vjp(m, d, qpos_t, qvel_t, lam_next, gadd, lam, grad_ctrl, rcap, smem, t);
 }
 __global__ void __launch_bounds__(64) k_adjoint_state_grad(mjhModel m, mjhData d, const float* lam, float* g_qpos,
                                                            float* g_qvel, float* g_act) {
@@ -1187,6 +1195,29 @@ int mjh_adjoint_step(const mjhModel* m, const mjhData* d, const float* A, const 
   hipLaunchKernelGGL(k_adjoint_step, dim3(d->nworld), dim3(64), 0, (hipStream_t)stream, *m, *d, A, B, g_qpos, g_qvel,
                      g_act, lam_next, lam, grad_ctrl);
   return check_launch("adjoint_step");
+}
+int mjh_adjoint_vjp(const mjhModel* m, const mjhData* d, const float* qpos, const float* qvel, const float* lam_next,
+                    const float* gadd, float* lam, float* grad_ctrl, mjhStream stream) {
+  if (!m || !d) return fail(MJH_ERR_INVALID, "null model or data");
+  if (!qpos || !qvel || !lam_next || !lam || (m->nu > 0 && !grad_ctrl))
+    return fail(MJH_ERR_INVALID, "adjoint_vjp: qpos, qvel, lam_next, lam (and grad_ctrl with actuators) must be set");
+  if (m->na > 0) return fail(MJH_ERR_UNSUPPORTED, "adjoint_vjp: activation states (na > 0) are not differentiated");
+  if (m->opt.integrator != mjhINT_EULER && m->opt.integrator != mjhINT_IMPLICITFAST)
+    return fail(MJH_ERR_UNSUPPORTED, "adjoint_vjp: integrator must be Euler or implicitfast");
+  if (m->actuator_affine_bias_gain && !(m->opt.disableflags & mjhDSBL_ACTUATION))
+    return fail(MJH_ERR_UNSUPPORTED, "adjoint_vjp: affine actuator gains / biases are not differentiated");
+  if (d->nworld <= 0) return MJH_OK;
+  const int fixed = lds_adjoint_floats(m->nbody, m->nv, 0);
+  const int cap = (int)(lds_cap() / sizeof(float));
+  // rows per world the LDS holds: the pool's per-world share, at most what fits
+  int rcap = d->njmax / d->nworld + 1;
+  if (rcap > (cap - fixed) / 8) rcap = (cap - fixed) / 8;
+  if (rcap < 4) return fail(MJH_ERR_CAPACITY, "adjoint_vjp: nv too large for the LDS of one workgroup");
+  const size_t bytes = (size_t)lds_adjoint_floats(m->nbody, m->nv, rcap) * sizeof(float);
+  TRY(allow_lds((const void*)k_adjoint_vjp, bytes, "adjoint_vjp"));
+  hipLaunchKernelGGL(k_adjoint_vjp, dim3(d->nworld), dim3(64), bytes, (hipStream_t)stream, *m, *d, qpos, qvel,
+                     lam_next, gadd, lam, grad_ctrl, rcap);
+  return check_launch("adjoint_vjp");
 }
 int mjh_adjoint_state_grad(const mjhModel* m, const mjhData* d, const float* lam, float* g_qpos, float* g_qvel,
                            float* g_act, mjhStream stream) {

@@ -94,6 +94,44 @@ def adjoint_step(m: Model, d: Data, A, B, lam, g_qpos=None, g_qvel=None, g_act=N
                _ptr(lam), _ptr(grad_ctrl))
 
 
+def vjp_step(m: Model, d: Data, qpos, qvel, lam_next, lam, grad_ctrl, gadd=None):
+  """The analytic vector-Jacobian product of the step just taken from
+  (qpos, qvel) [nworld, nq] / [nworld, nv]: d holds that step's forward
+  quantities (as `step_fused` leaves them).  lam = (dx'/dx)' lam_next (+
+  gadd), grad_ctrl = (dx'/du)' lam_next."""
+  _native.call("mjh_adjoint_vjp", m, d, _ptr(qpos), _ptr(qvel), _ptr(lam_next), _ptr(gadd), _ptr(lam),
+               _ptr(grad_ctrl))
+
+
+def check_analytic(mjm):
+  """Raises NotImplementedError if the analytic adjoint does not cover the
+  model (mjh_adjoint_vjp: Euler / implicitfast, free / hinge / slide joints,
+  no springs on free joints, no activation states, joint transmissions with
+  fixed gains, no actuator force clamps at the joints)."""
+  import numpy as np
+
+  if int(mjm.opt.integrator) not in (0, 3):
+    raise NotImplementedError("analytic adjoint: integrator must be Euler or implicitfast")
+  if int(mjm.na) > 0:
+    raise NotImplementedError("analytic adjoint: activation states are not differentiated")
+  jt = np.asarray(mjm.jnt_type)
+  if np.any(jt == 1):
+    raise NotImplementedError("analytic adjoint: ball joints are not differentiated")
+  if np.any((jt == 0) & (np.asarray(mjm.jnt_stiffness) != 0)):
+    raise NotImplementedError("analytic adjoint: free-joint springs are not differentiated")
+  if int(mjm.nu):
+    if np.any(np.asarray(mjm.actuator_gaintype) != 0) or np.any(np.asarray(mjm.actuator_biastype) != 0):
+      raise NotImplementedError("analytic adjoint: actuators must have fixed gains and no bias")
+    if np.any(np.asarray(mjm.actuator_dyntype) != 0):
+      raise NotImplementedError("analytic adjoint: actuator dynamics are not differentiated")
+    trn = np.asarray(mjm.actuator_trntype)
+    tid = np.asarray(mjm.actuator_trnid)[:, 0]
+    if np.any(trn != 0) or np.any(jt[tid] == 1):
+      raise NotImplementedError("analytic adjoint: actuators must act on hinge, slide or free joints")
+  if hasattr(mjm, "jnt_actfrclimited") and np.any(np.asarray(mjm.jnt_actfrclimited)):
+    raise NotImplementedError("analytic adjoint: joint actuator force limits are not differentiated")
+
+
 def adjoint_state_grad(m: Model, d: Data, lam, g_qpos, g_qvel, g_act=None):
   """Splits lam [nworld, nx] into gradients with respect to d's raw qpos,
   qvel and act."""
@@ -104,33 +142,60 @@ class Rollout:
   """Differentiable `nstep`-step rollout of `nworld` worlds:
   `qpos, qvel = r(qpos0, qvel0, ctrl)` with qpos [nstep, nworld, nq] and
   qvel [nstep, nworld, nv] the states after each step.  Inputs other than
-  (qpos, qvel, ctrl) -- qfrc_applied, xfrc_applied, mocap -- are read from
-  `r.d`, set them there; act and qacc_warmstart start from `act0` /
-  `warmstart0` (zeros by default)."""
+  (qpos, qvel, ctrl) -- qfrc_applied, mocap -- are read from `r.d`, set
+  them there; act and qacc_warmstart start from `act0` / `warmstart0`
+  (zeros by default).
+
+  adjoint="analytic" (the default) runs the reverse pass with the
+  hand-written step adjoint (mjh_adjoint_vjp): each step of the backward
+  pass recomputes that step's forward quantities from its saved input
+  state (one fused step) and takes one vector-Jacobian product.
+  adjoint="fd" uses finite-difference transition Jacobians instead
+  (TransitionFD; any model the step runs).  checkpoint_every=k keeps the
+  input state of every k-th step only (memory / k) and recomputes each
+  segment of k steps from its checkpoint in the backward pass (one more
+  step per step)."""
 
   def __init__(self, mjm, m: Model, nworld: int, nstep: int, nconmax: int = -1, njmax: int = -1, eps: float = 4e-3,
-               centered: bool = True, device=None):
+               centered: bool = True, device=None, adjoint: str = "analytic", checkpoint_every: int = 1):
+    if adjoint not in ("analytic", "fd"):
+      raise ValueError(f"adjoint must be 'analytic' or 'fd', got {adjoint!r}")
+    if adjoint == "analytic":
+      check_analytic(mjm)
     self.m = m
+    self.adjoint = adjoint
     self.nworld, self.nstep = int(nworld), int(nstep)
+    self.every = max(1, int(checkpoint_every))
     self.d = io.make_data(mjm, nworld=nworld, nconmax=nconmax, njmax=njmax, device=device)
-    self.tfd = TransitionFD(mjm, m, nworld, nconmax, njmax, eps, centered, device)
+    self.tfd = TransitionFD(mjm, m, nworld, nconmax, njmax, eps, centered, device) if adjoint == "fd" else None
     self.nq, self.nv, self.na, self.nu = int(mjm.nq), int(mjm.nv), int(mjm.na), int(mjm.nu)
     dev = self.d.qpos.device
     z = lambda *s: torch.zeros(s, dtype=torch.float32, device=dev)
     T, W = self.nstep, self.nworld
-    # inputs of each step (checkpoints)
-    self.s_qpos, self.s_qvel = z(T, W, self.nq), z(T, W, self.nv)
-    self.s_act, self.s_warm = z(T, W, self.na), z(T, W, self.nv)
+    # input states of the checkpointed steps, and of one segment's steps
+    C = (T + self.every - 1) // self.every
+    self.c_qpos, self.c_qvel = z(C, W, self.nq), z(C, W, self.nv)
+    self.c_act, self.c_warm = z(C, W, self.na), z(C, W, self.nv)
+    S = self.every
+    self.s_qpos, self.s_qvel = z(S, W, self.nq), z(S, W, self.nv)
+    self.s_act, self.s_warm = z(S, W, self.na), z(S, W, self.nv)
     self.lam = z(W, nx_of(mjm))
     self.lam_next = z(W, nx_of(mjm))
+    self.lam_g = z(W, nx_of(mjm))
 
-  def _load(self, t, ctrl):
+  def _save(self, buf, i):
     d = self.d
-    d.qpos.copy_(self.s_qpos[t])
-    d.qvel.copy_(self.s_qvel[t])
-    d.act.copy_(self.s_act[t])
-    d.qacc_warmstart.copy_(self.s_warm[t])
-    d.ctrl.copy_(ctrl[t])
+    buf[0][i].copy_(d.qpos)
+    buf[1][i].copy_(d.qvel)
+    buf[2][i].copy_(d.act)
+    buf[3][i].copy_(d.qacc_warmstart)
+
+  def _restore(self, buf, i):
+    d = self.d
+    d.qpos.copy_(buf[0][i])
+    d.qvel.copy_(buf[1][i])
+    d.act.copy_(buf[2][i])
+    d.qacc_warmstart.copy_(buf[3][i])
 
   def forward(self, qpos0, qvel0, ctrl, act0=None, warmstart0=None):
     """Runs the rollout without recording anything for autograd."""
@@ -143,11 +208,10 @@ class Rollout:
     d.qacc_warmstart.copy_(warmstart0) if warmstart0 is not None else d.qacc_warmstart.zero_()
     qpos = torch.empty((T, self.nworld, self.nq), dtype=torch.float32, device=d.qpos.device)
     qvel = torch.empty((T, self.nworld, self.nv), dtype=torch.float32, device=d.qpos.device)
+    ck = (self.c_qpos, self.c_qvel, self.c_act, self.c_warm)
     for t in range(T):
-      self.s_qpos[t].copy_(d.qpos)
-      self.s_qvel[t].copy_(d.qvel)
-      self.s_act[t].copy_(d.act)
-      self.s_warm[t].copy_(d.qacc_warmstart)
+      if t % self.every == 0:
+        self._save(ck, t // self.every)
       d.ctrl.copy_(ctrl[t])
       api.step_fused(self.m, d)
       qpos[t].copy_(d.qpos)
@@ -157,19 +221,45 @@ class Rollout:
   def backward(self, ctrl, qpos, g_qpos, g_qvel):
     """Gradients (qpos0, qvel0, ctrl) from the gradients with respect to the
     returned trajectories (the forward pass must have run with ctrl)."""
-    m, d, T = self.m, self.d, self.nstep
+    m, d, T, K = self.m, self.d, self.nstep, self.every
+    if self.adjoint == "analytic" and bool(torch.any(d.xfrc_applied != 0)):
+      raise NotImplementedError("analytic adjoint: xfrc_applied is not differentiated; use adjoint='fd'")
     grad_ctrl = torch.zeros_like(ctrl)
+    ck = (self.c_qpos, self.c_qvel, self.c_act, self.c_warm)
+    seg = (self.s_qpos, self.s_qvel, self.s_act, self.s_warm)
     # lam_T = g(x_T), at x_T's qpos
     d.qpos.copy_(qpos[T - 1])
     adjoint_step(m, d, None, None, self.lam_next, g_qpos=g_qpos[T - 1], g_qvel=g_qvel[T - 1])
-    for t in range(T - 1, -1, -1):
-      self._load(t, ctrl)
-      A, B = self.tfd(d)
-      g_q = g_qpos[t - 1] if t > 0 else None
-      g_v = g_qvel[t - 1] if t > 0 else None
-      adjoint_step(m, d, A, B, self.lam, g_qpos=g_q, g_qvel=g_v, lam_next=self.lam_next, grad_ctrl=grad_ctrl[t])
-      self.lam, self.lam_next = self.lam_next, self.lam
-    # lam_next now holds lam_0; d holds x_0
+    for s0 in range(((T - 1) // K) * K, -1, -K):
+      s1 = min(s0 + K, T)
+      # the segment's input states, recomputed from its checkpoint
+      self._restore(ck, s0 // K)
+      for t in range(s0, s1):
+        self._save(seg, t - s0)
+        if t + 1 < s1:
+          d.ctrl.copy_(ctrl[t])
+          api.step_fused(m, d)
+      for t in range(s1 - 1, s0 - 1, -1):
+        i = t - s0
+        self._restore(seg, i)
+        d.ctrl.copy_(ctrl[t])
+        g_q = g_qpos[t - 1] if t > 0 else None
+        g_v = g_qvel[t - 1] if t > 0 else None
+        if self.adjoint == "fd":
+          A, B = self.tfd(d)
+          adjoint_step(m, d, A, B, self.lam, g_qpos=g_q, g_qvel=g_v, lam_next=self.lam_next,
+                       grad_ctrl=grad_ctrl[t])
+        else:
+          gadd = None
+          if t > 0:
+            adjoint_step(m, d, None, None, self.lam_g, g_qpos=g_q, g_qvel=g_v)
+            gadd = self.lam_g
+          api.step_fused(m, d)
+          vjp_step(m, d, seg[0][i], seg[1][i], self.lam_next, self.lam, grad_ctrl[t], gadd)
+        self.lam, self.lam_next = self.lam_next, self.lam
+    # lam_next now holds lam_0; x_0's qpos turns its rotation part into
+    # raw quaternion gradients
+    d.qpos.copy_(self.c_qpos[0])
     gq0 = torch.empty((self.nworld, self.nq), dtype=torch.float32, device=d.qpos.device)
     gv0 = torch.empty((self.nworld, self.nv), dtype=torch.float32, device=d.qpos.device)
     ga0 = torch.empty((self.nworld, self.na), dtype=torch.float32, device=d.qpos.device)
