@@ -68,8 +68,11 @@ def kernel_sources_sha() -> str:
   import hashlib
 
   h = hashlib.sha256()
-  for path in sorted(glob.glob(os.path.join(ROOT, "mjx_warp_amd", "csrc", "*.hpp"))
-                     + [os.path.join(ROOT, "mjx_warp_amd", "csrc", "kernels.hip")]):
+  # adjoint.hpp holds the step adjoint's kernel (built from api.hip only),
+  # not a stage of the step
+  hpp = [p for p in glob.glob(os.path.join(ROOT, "mjx_warp_amd", "csrc", "*.hpp"))
+         if os.path.basename(p) != "adjoint.hpp"]
+  for path in sorted(hpp) + [os.path.join(ROOT, "mjx_warp_amd", "csrc", "kernels.hip")]:
     with open(path, "rb") as f:
       h.update(os.path.basename(path).encode() + b"\0" + f.read())
   return h.hexdigest()[:16]
@@ -205,13 +208,15 @@ def time_rollout_graph(args, nstep: int = 10):
           "steps_per_graph": nstep, "replays": reps}
 
 
-def time_adjoint(args, nworld: int = 1024, nstep: int = 50, eps: float = 4e-3, reps: int = 2):
+def time_adjoint(args, nworld: int = 1024, nstep: int = 50, eps: float = 4e-3, reps: int = 2, fd: bool = True):
   """Forward + adjoint world-steps/s of the differentiable rollout: the
   humanoid (gear 100, implicitfast, Newton) from perturbed squat states,
   nworld worlds, nstep steps, ctrl ~ U(-1, 1) [nstep, nworld, 21], loss
   sum_t (torso z - 1.3)^2 + 1e-3 |ctrl|^2, gradients with respect to ctrl and
-  qvel0 through the finite-difference transition Jacobians (centered, eps)
-  and the adjoint recursion."""
+  qvel0.  The value is the analytic adjoint (each backward step recomputes
+  its forward quantities with one fused step and takes one vector-Jacobian
+  product); the finite-difference adjoint (centered transition Jacobians,
+  eps) is timed beside it."""
   import torch
 
   import mjx_warp_amd as mjw
@@ -219,7 +224,6 @@ def time_adjoint(args, nworld: int = 1024, nstep: int = 50, eps: float = 4e-3, r
 
   mjm = models.humanoid(gear=100.0, iterations=args.iterations, ls_iterations=args.ls_iterations)
   m = mjw.put_model(mjm)
-  r = derivative.Rollout(mjm, m, nworld, nstep, nconmax=64 * nworld, njmax=277 * nworld, eps=eps)
   q0, v0 = models.humanoid_rollout_state(mjm, nworld)
   gen = torch.Generator(device="cuda")
   gen.manual_seed(1)
@@ -227,39 +231,51 @@ def time_adjoint(args, nworld: int = 1024, nstep: int = 50, eps: float = 4e-3, r
   tq0 = torch.tensor(q0, dtype=torch.float32, device="cuda")
   tv0 = torch.tensor(v0, dtype=torch.float32, device="cuda")
 
-  def fwd_adj():
-    qv = tv0.clone().requires_grad_(True)
-    u = ctrl.clone().requires_grad_(True)
-    qpos, _ = r(tq0, qv, u)
-    loss = ((qpos[..., 2] - 1.3) ** 2).sum() + 1e-3 * (u**2).sum()
-    loss.backward()
-    return u.grad, qv.grad
+  def measure(r):
+    def fwd_adj():
+      qv = tv0.clone().requires_grad_(True)
+      u = ctrl.clone().requires_grad_(True)
+      qpos, _ = r(tq0, qv, u)
+      loss = ((qpos[..., 2] - 1.3) ** 2).sum() + 1e-3 * (u**2).sum()
+      loss.backward()
+      return u.grad, qv.grad
 
-  def fwd():
-    with torch.no_grad():
-      r.forward(tq0, tv0, ctrl)
+    def fwd():
+      with torch.no_grad():
+        r.forward(tq0, tv0, ctrl)
 
-  fwd_adj()
-  fwd()
-  torch.cuda.synchronize()
-  t0 = time.perf_counter()
-  for _ in range(reps):
+    fwd_adj()
     fwd()
-  torch.cuda.synchronize()
-  t_fwd = (time.perf_counter() - t0) / reps
-  t0 = time.perf_counter()
-  for _ in range(reps):
-    gu, gv = fwd_adj()
-  torch.cuda.synchronize()
-  t_all = (time.perf_counter() - t0) / reps
-  finite = bool(torch.isfinite(gu).all().item() and torch.isfinite(gv).all().item())
-  return {"value": nworld * nstep / t_all, "unit": "world-steps/s", "forward_value": nworld * nstep / t_fwd,
-          "cost_ratio": t_all / t_fwd, "ms_forward": 1000.0 * t_fwd, "ms_forward_adjoint": 1000.0 * t_all,
-          "nworld": nworld, "nstep": nstep, "eps": eps, "fd_worlds": nworld * r.tfd.per_world,
-          "gradients_finite": finite,
-          "workload": "humanoid 27-dof rollout, gradients of sum_t (z_t - 1.3)^2 + 1e-3 |u_t|^2 w.r.t. "
-                      "ctrl [50, 1024, 21] and qvel0 [1024, 27]; centered finite-difference transition "
-                      "Jacobians (eps 4e-3) + adjoint recursion"}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+      fwd()
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+      gu, gv = fwd_adj()
+    torch.cuda.synchronize()
+    t_all = (time.perf_counter() - t0) / reps
+    finite = bool(torch.isfinite(gu).all().item() and torch.isfinite(gv).all().item())
+    return {"value": nworld * nstep / t_all, "forward_value": nworld * nstep / t_fwd, "cost_ratio": t_all / t_fwd,
+            "ms_forward": 1000.0 * t_fwd, "ms_forward_adjoint": 1000.0 * t_all, "gradients_finite": finite}, gu
+
+  ra = derivative.Rollout(mjm, m, nworld, nstep, nconmax=64 * nworld, njmax=277 * nworld)
+  out, gu_a = measure(ra)
+  del ra
+  out.update({"unit": "world-steps/s", "nworld": nworld, "nstep": nstep, "adjoint": "analytic",
+              "workload": "humanoid 27-dof rollout, gradients of sum_t (z_t - 1.3)^2 + 1e-3 |u_t|^2 w.r.t. "
+                          "ctrl [50, 1024, 21] and qvel0 [1024, 27]; analytic step adjoint (recompute + "
+                          "vector-Jacobian product per step, every step's input kept)"})
+  if fd:
+    rf = derivative.Rollout(mjm, m, nworld, nstep, nconmax=64 * nworld, njmax=277 * nworld, eps=eps, adjoint="fd")
+    f, gu_f = measure(rf)
+    f.update({"eps": eps, "fd_worlds": nworld * rf.tfd.per_world,
+              "ctrl_grad_rel_diff_to_analytic": float(torch.linalg.norm(gu_f - gu_a) / torch.linalg.norm(gu_a))})
+    out["finite_difference"] = f
+    del rf
+  return out
 
 
 def time_heterogeneous(args, nworld: int = 16384, steps: int = 50, graph_steps: int = 10):
@@ -653,7 +669,9 @@ def main():
   adj = None
   if not args.no_adjoint:
     adj = time_adjoint(args)
-    log(f"forward+adjoint: {adj['value']:.4g} world-steps/s (cost ratio {adj['cost_ratio']:.1f})")
+    log(f"forward+adjoint: {adj['value']:.4g} world-steps/s (cost ratio {adj['cost_ratio']:.2f}, analytic)"
+        + (f"; finite differences {adj['finite_difference']['value']:.4g} (cost ratio "
+           f"{adj['finite_difference']['cost_ratio']:.1f})" if "finite_difference" in adj else ""))
   bytes_ws = algorithmic_bytes(mjm, fused["ncon_w"], fused["nefc_w"])
   achieved = bytes_ws * args.batch / (fused["kernel_ms"] * 1e-3) / 1e9
   base = None

@@ -53,10 +53,11 @@
 
 namespace mjh {
 
-// LDS floats of adjoint_vjp: per body 6 six-vectors, per dof 6 prefixes,
-// 7 nv-vectors (one of them integer), the nv x nv matrix, 8 per row.
+// LDS floats of adjoint_vjp: per body 6 six-vectors and the com-frame
+// inertia, per dof 6 prefixes, cdof and cdof_dot, 7 nv-vectors (one of them
+// integer), the nv x nv matrix, 8 per row.
 __host__ __device__ inline int lds_adjoint_floats(int nbody, int nv, int rows) {
-  return 36 * nbody + 36 * nv + 7 * nv + nv * nv + 8 * rows;
+  return 46 * nbody + 48 * nv + 7 * nv + nv * nv + 8 * rows;
 }
 
 // k, b, impedance and its derivative with respect to the row's distance,
@@ -137,9 +138,9 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
   const float* vt = qvel_t_all + (size_t)w * nv;
   const float* v1 = d.qvel + (size_t)w * nv;
   const float* ln = lam_next + (size_t)w * nx;
-  const float* cdof = WPTR(d.cdof, w, nv * 6);
-  const float* cdot = WPTR(d.cdof_dot, w, nv * 6);
-  const float* cinert = WPTR(d.cinert, w, nb * 10);
+  const float* cdof_g = WPTR(d.cdof, w, nv * 6);
+  const float* cdot_g = WPTR(d.cdof_dot, w, nv * 6);
+  const float* cinert_g = WPTR(d.cinert, w, nb * 10);
   const float* sc = WPTR(d.subtree_com, w, nb * 3);
   const float* qM = WPTR(d.qMd, w, nv * nv);
   const float* a_s = WPTR(d.qacc, w, nv);
@@ -174,6 +175,16 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
   float* rbe = ral + rcap;
   float* rga = rbe + rcap;
   int* rsrc = (int*)(rga + rcap);
+  // the motion axes, their derivatives and the com-frame inertias, read
+  // many times by the per-dof loops below
+  float* cdof = (float*)(rsrc + rcap);
+  float* cdot = cdof + 6 * nv;
+  float* cinert = cdot + 6 * nv;
+  for (int i = t.lane; i < 6 * nv; i += TEAM) {
+    cdof[i] = cdof_g[i];
+    cdot[i] = cdot_g[i];
+  }
+  for (int i = t.lane; i < 10 * nb; i += TEAM) cinert[i] = cinert_g[i];
 
   // ---- integration: qpos' = qpos (+) dt qvel', qvel' = qvel + dt a
   for (int i = t.lane; i < nv; i += TEAM) {

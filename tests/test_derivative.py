@@ -202,7 +202,7 @@ def test_transition_fd_humanoid_matches_oracle():
 def _gpu_rollout_grads(mjm, q0, v0, ctrl, eps=4e-3):
   n, T = q0.shape[0], ctrl.shape[0]
   m = mjw.put_model(mjm)
-  r = derivative.Rollout(mjm, m, n, T, nconmax=64 * n, njmax=277 * n, eps=eps)
+  r = derivative.Rollout(mjm, m, n, T, nconmax=64 * n, njmax=277 * n, eps=eps, adjoint="fd")
   tq0 = torch.tensor(q0, dtype=torch.float32, device="cuda", requires_grad=True)
   tv0 = torch.tensor(v0, dtype=torch.float32, device="cuda", requires_grad=True)
   tu = torch.tensor(ctrl, dtype=torch.float32, device="cuda", requires_grad=True)
