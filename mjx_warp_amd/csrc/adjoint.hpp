@@ -55,9 +55,10 @@ namespace mjh {
 
 // LDS floats of adjoint_vjp: per body 6 six-vectors and the com-frame
 // inertia, per dof 6 prefixes, cdof and cdof_dot, 7 nv-vectors (one of them
-// integer), the nv x nv matrix, 8 per row.
+// integer), the nv x nv matrix and the register factor's two 32-float
+// broadcast buffers, 8 per row, and two per (dof, body) pair.
 __host__ __device__ inline int lds_adjoint_floats(int nbody, int nv, int rows) {
-  return 46 * nbody + 48 * nv + 7 * nv + nv * nv + 8 * rows;
+  return 46 * nbody + 48 * nv + 7 * nv + nv * nv + 64 + 8 * rows + 2 * nv * nbody;
 }
 
 // k, b, impedance and its derivative with respect to the row's distance,
@@ -167,7 +168,7 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
   float* tmp = wv + nv;
   int* blk = (int*)(tmp + nv);
   float* A = (float*)(blk + nv);
-  float* rlf = A + nv * nv;
+  float* rlf = A + nv * nv + 64;
   float* rg = rlf + rcap;
   float* rcq = rg + rcap;
   float* rcv = rcq + rcap;
@@ -221,18 +222,39 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
   }
   sync();
 
-  // ---- mu = M_hat^-1 (dt lam_qvel'), M_hat block diagonal by tree
-  for (int p = t.lane; p < nv * nv; p += TEAM) {
-    const int r = p / nv, c = p % nv;
-    float v = (c >= blk[r] && c < (int)tmp[r]) ? qM[p] : 0.0f;
-    if (r == c && damp) v += dt * m.dof_damping[r];
-    A[p] = v;
+  // ---- mu = M_hat^-1 (dt lam_qvel'), M_hat block diagonal by tree.
+  // nv <= 32: a row per lane in registers (RegChol; both 32-lane halves of
+  // the wave factor the same matrix, each with its own broadcast buffer);
+  // larger: the LDS factor
+  const bool reg = nv <= kRegNvMax;
+  const int rl = t.lane & 31;
+  float* cbuf = A + nv * nv + (t.lane & 32);
+  int rbeg = rl, rend = rl + 1;
+  if (reg && rl < nv) tree_block(m, rl, &rbeg, &rend);
+  if (reg) {
+    const float dg = (rl < nv && damp) ? dt * m.dof_damping[rl] : 0.0f;
+    const float b = rl < nv ? dt * lv[rl] : 0.0f;
+    const float x = with_nvc(nv, [&](auto nvc) {
+      RegChol<decltype(nvc)::value> c{rl, cbuf};
+      c.load_sym(qM, nv, rbeg, rend, dg);
+      c.factor();
+      return c.solve(b);
+    });
+    if (t.lane < nv) mu[t.lane] = x;
+    sync();
+  } else {
+    for (int p = t.lane; p < nv * nv; p += TEAM) {
+      const int r = p / nv, c = p % nv;
+      float v = (c >= blk[r] && c < (int)tmp[r]) ? qM[p] : 0.0f;
+      if (r == c && damp) v += dt * m.dof_damping[r];
+      A[p] = v;
+    }
+    for (int i = t.lane; i < nv; i += TEAM) mu[i] = dt * lv[i];
+    sync();
+    chol_lds(A, nv, nv, m, t);
+    cho_solve_lds(A, nv, nv, mu, t);
+    sync();
   }
-  for (int i = t.lane; i < nv; i += TEAM) mu[i] = dt * lv[i];
-  sync();
-  chol_lds(A, nv, nv, m, t);
-  cho_solve_lds(A, nv, nv, mu, t);
-  sync();
 
   // ---- constraint rows of this world
   int base = 0, n = 0;
@@ -262,20 +284,41 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
       if (fr[i] > 0.0f) s -= J[(size_t)i * nv + j] * Dr[i] * rlf[i];
     nuv[j] = s;
   }
-  for (int p = t.lane; p < nv * nv; p += TEAM) {
-    const int r = p / nv, c = p % nv;
-    if (c > r) continue;
-    float v = (c >= blk[r] && c < (int)tmp[r]) ? qM[p] : 0.0f;
-    for (int i = 0; i < n; i++)
-      if (fr[i] > 0.0f) v += Dr[i] * J[(size_t)i * nv + r] * J[(size_t)i * nv + c];
-    A[p] = v;
-  }
   sync();
-  if (n > 0) {
+  if (n == 0) {
+    for (int i = t.lane; i < nv; i += TEAM) nuv[i] = 0.0f;
+  } else if (reg) {
+    // the lane's row of H: its block of M, then D_i J_i[row] J_i per active row
+    const float b = rl < nv ? nuv[rl] : 0.0f;
+    const float x = with_nvc(nv, [&](auto nvc) {
+      constexpr int NVC = decltype(nvc)::value;
+      RegChol<NVC> c{rl, cbuf};
+      c.load_sym(qM, nv, rbeg, rend, 0.0f);
+      for (int i = 0; i < n; i++) {
+        if (!(fr[i] > 0.0f)) continue;
+        const float* Ji = J + (size_t)i * nv;
+        const float g = rl < nv ? Dr[i] * Ji[rl] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < NVC; j++)
+          if (j < nv) c.h[j] = fmaf(g, Ji[j], c.h[j]);
+      }
+      c.factor();
+      return c.solve(b);
+    });
+    sync();
+    if (t.lane < nv) nuv[t.lane] = x;
+  } else {
+    for (int p = t.lane; p < nv * nv; p += TEAM) {
+      const int r = p / nv, c = p % nv;
+      if (c > r) continue;
+      float v = (c >= blk[r] && c < (int)tmp[r]) ? qM[p] : 0.0f;
+      for (int i = 0; i < n; i++)
+        if (fr[i] > 0.0f) v += Dr[i] * J[(size_t)i * nv + r] * J[(size_t)i * nv + c];
+      A[p] = v;
+    }
+    sync();
     chol_lds(A, nv, nv, m, t);
     cho_solve_lds(A, nv, nv, nuv, t);
-  } else {
-    for (int i = t.lane; i < nv; i += TEAM) nuv[i] = 0.0f;
   }
   sync();
   for (int i = t.lane; i < n; i += TEAM) {
@@ -453,48 +496,69 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
   }
   sync();
 
-  // ---- configuration and velocity derivatives, lane per dof k
-  for (int k = t.lane; k < nv; k += TEAM) {
-    const S6 S = ld6(cdof + 6 * k);
+  // ---- configuration and velocity derivatives of rne, a lane per (dof k,
+  // body b) pair into LDS, then summed per dof in body order (the same sum
+  // every run)
+  float* pq = cinert + 10 * nb;
+  float* pw = pq + nv * nb;
+  for (int pr = t.lane; pr < nv * nb; pr += TEAM) {
+    const int k = pr / nb, b = pr - k * nb;
     const int kb = m.dof_bodyid[k];
+    pq[pr] = 0.0f;
+    pw[pr] = 0.0f;
+    if (b < kb || !m.body_dof_tree[b * nv + k]) continue;
+    const S6 S = ld6(cdof + 6 * k);
     const int kj = m.dof_jntid[k];
     const bool ktrans = m.jnt_type[kj] == mjhJNT_FREE && k - m.jnt_dofadr[kj] < 3;
-    const S6 Uv = ld6(pv + 6 * k), UaA = ld6(paA + 6 * k), UaS = ld6(paS + 6 * k);
-    const S6 Um = ld6(pm + 6 * k), Un = ld6(pn + 6 * k), Ua = ld6(pa + 6 * k);
+    const S6 Uv = ld6(pv + 6 * k);
     const S6 SUv = motion_cross(S, Uv);
     // velocity derivative: the prefix through k, and k's own cdof_dot
     const S6 P = (m.jnt_type[kj] == mjhJNT_FREE && !ktrans) ? ld6(cv + 6 * kb) : Uv;
     const S6 cdk = ktrans ? S6{{0.f, 0.f, 0.f, 0.f, 0.f, 0.f}} : ld6(cdot + 6 * k);
-    float accq = 0.0f, accv = 0.0f;
-    for (int b = kb; b < nb; b++) {
-      if (!m.body_dof_tree[b * nv + k]) continue;
-      const float* I = cinert + 10 * b;
-      const S6 cvb = ld6(cv + 6 * b);
-      const S6 Iv = inert_vec(I, cvb);
-      const S6 dvv = motion_cross(S, s6_sub(cvb, Uv));
-      const S6 dIcv = inert_moved(I, S, cvb);
-      const S6 gyro = s6_add(motion_cross_force(dvv, Iv), motion_cross_force(cvb, s6_add(dIcv, inert_vec(I, dvv))));
-      const S6 cvI = motion_cross_force(cvb, Iv);
-      // (a, mu) and (a_s, nu)
+    float accq = 0.0f;
+    const float* I = cinert + 10 * b;
+    const S6 cvb = ld6(cv + 6 * b);
+    const S6 Iv = inert_vec(I, cvb);
+    const S6 dvv = motion_cross(S, s6_sub(cvb, Uv));
+    const S6 dIcv = inert_moved(I, S, cvb);
+    const S6 gyro = s6_add(motion_cross_force(dvv, Iv), motion_cross_force(cvb, s6_add(dIcv, inert_vec(I, dvv))));
+    const S6 cvI = motion_cross_force(cvb, Iv);
+    // (a, mu) and (a_s, nu)
 #pragma unroll
-      for (int which = 0; which < 2; which++) {
-        const S6 ca = ld6((which == 0 ? caA : caS) + 6 * b);
-        const S6 Uac = which == 0 ? UaA : UaS;
-        const S6 V = ld6((which == 0 ? Vm : Vn) + 6 * b);
-        const S6 UV = which == 0 ? Um : Un;
-        const S6 dca = s6_sub(motion_cross(S, s6_sub(ca, Uac)), motion_cross(SUv, s6_sub(cvb, Uv)));
-        const S6 fb = s6_add(inert_vec(I, ca), cvI);
-        const S6 df = s6_add(s6_add(inert_moved(I, S, ca), inert_vec(I, dca)), gyro);
-        const S6 dV = motion_cross(S, s6_sub(V, UV));
-        accq += dot6(df, V) + dot6(fb, dV);
-      }
-      // bias force along qvel_k, left vector mu + nu
-      const S6 Vw = s6_add(ld6(Vm + 6 * b), ld6(Vn + 6 * b));
-      const S6 dca = s6_add(motion_cross(S, s6_sub(cvb, P)), cdk);
-      const S6 df = s6_add(s6_add(inert_vec(I, dca), motion_cross_force(S, Iv)),
-                           motion_cross_force(cvb, inert_vec(I, S)));
-      accv += dot6(df, Vw);
+    for (int which = 0; which < 2; which++) {
+      const S6 ca = ld6((which == 0 ? caA : caS) + 6 * b);
+      const S6 Uac = ld6((which == 0 ? paA : paS) + 6 * k);
+      const S6 V = ld6((which == 0 ? Vm : Vn) + 6 * b);
+      const S6 UV = ld6((which == 0 ? pm : pn) + 6 * k);
+      const S6 dca = s6_sub(motion_cross(S, s6_sub(ca, Uac)), motion_cross(SUv, s6_sub(cvb, Uv)));
+      const S6 fb = s6_add(inert_vec(I, ca), cvI);
+      const S6 df = s6_add(s6_add(inert_moved(I, S, ca), inert_vec(I, dca)), gyro);
+      const S6 dV = motion_cross(S, s6_sub(V, UV));
+      accq += dot6(df, V) + dot6(fb, dV);
     }
+    // bias force along qvel_k, left vector mu + nu
+    const S6 Vw = s6_add(ld6(Vm + 6 * b), ld6(Vn + 6 * b));
+    const S6 dcv = s6_add(motion_cross(S, s6_sub(cvb, P)), cdk);
+    const S6 dfv = s6_add(s6_add(inert_vec(I, dcv), motion_cross_force(S, Iv)),
+                          motion_cross_force(cvb, inert_vec(I, S)));
+    pq[pr] = accq;
+    pw[pr] = dot6(dfv, Vw);
+  }
+  sync();
+  for (int k = t.lane; k < nv; k += TEAM) {
+    float sq = 0.0f, sw = 0.0f;
+    for (int b = 0; b < nb; b++) {
+      sq += pq[k * nb + b];
+      sw += pw[k * nb + b];
+    }
+    tmp[k] = sq;
+    wv[k] = sw;
+  }
+  sync();
+  // ---- contact rows moving with the configuration, lane per dof k
+  for (int k = t.lane; k < nv; k += TEAM) {
+    const S6 S = ld6(cdof + 6 * k);
+    const S6 Uv = ld6(pv + 6 * k), Um = ld6(pm + 6 * k), Un = ld6(pn + 6 * k), Ua = ld6(pa + 6 * k);
     // contact Jacobian rows moving with the configuration
     float accc = 0.0f;
     for (int i = 0; i < n; i++) {
@@ -624,8 +688,8 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
       // contact point: d . (omega_rel x dp)
       accc += dot(dp, cross(dir, om));
     }
-    lq[k] += accc - accq;
-    lv[k] -= accv;
+    lq[k] += accc - tmp[k];
+    lv[k] -= wv[k];
   }
   sync();
   float* lo = lam + (size_t)w * nx;
