@@ -241,8 +241,9 @@ def time_adjoint(args, nworld: int = 1024, nstep: int = 50, eps: float = 4e-3, r
       return u.grad, qv.grad
 
     def fwd():
+      # the rollout alone: no tape writes
       with torch.no_grad():
-        r.forward(tq0, tv0, ctrl)
+        r.forward(tq0, tv0, ctrl, record=False)
 
     fwd_adj()
     fwd()
@@ -263,11 +264,18 @@ def time_adjoint(args, nworld: int = 1024, nstep: int = 50, eps: float = 4e-3, r
 
   ra = derivative.Rollout(mjm, m, nworld, nstep, nconmax=64 * nworld, njmax=277 * nworld)
   out, gu_a = measure(ra)
+  taped = ra.tape is not None
+  tape_mb = ra.tape.numel() / 2**20 if taped else 0.0
   del ra
   out.update({"unit": "world-steps/s", "nworld": nworld, "nstep": nstep, "adjoint": "analytic",
+              "taped": taped, "tape_mib": tape_mb,
               "workload": "humanoid 27-dof rollout, gradients of sum_t (z_t - 1.3)^2 + 1e-3 |u_t|^2 w.r.t. "
-                          "ctrl [50, 1024, 21] and qvel0 [1024, 27]; analytic step adjoint (recompute + "
-                          "vector-Jacobian product per step, every step's input kept)"})
+                          "ctrl [50, 1024, 21] and qvel0 [1024, 27]; analytic step adjoint (one "
+                          "vector-Jacobian product per step, each step's fields taped in the forward pass)"})
+  rr = derivative.Rollout(mjm, m, nworld, nstep, nconmax=64 * nworld, njmax=277 * nworld, tape=False)
+  r2, _ = measure(rr)
+  del rr
+  out["recompute"] = {k: r2[k] for k in ("value", "cost_ratio", "ms_forward_adjoint")}
   if fd:
     rf = derivative.Rollout(mjm, m, nworld, nstep, nconmax=64 * nworld, njmax=277 * nworld, eps=eps, adjoint="fd")
     f, gu_f = measure(rf)
@@ -669,7 +677,8 @@ def main():
   adj = None
   if not args.no_adjoint:
     adj = time_adjoint(args)
-    log(f"forward+adjoint: {adj['value']:.4g} world-steps/s (cost ratio {adj['cost_ratio']:.2f}, analytic)"
+    log(f"forward+adjoint: {adj['value']:.4g} world-steps/s (cost ratio {adj['cost_ratio']:.2f}, analytic, "
+        f"taped; recomputing {adj['recompute']['value']:.4g}, cost ratio {adj['recompute']['cost_ratio']:.2f})"
         + (f"; finite differences {adj['finite_difference']['value']:.4g} (cost ratio "
            f"{adj['finite_difference']['cost_ratio']:.1f})" if "finite_difference" in adj else ""))
   bytes_ws = algorithmic_bytes(mjm, fused["ncon_w"], fused["nefc_w"])

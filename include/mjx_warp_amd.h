@@ -568,6 +568,17 @@ int mjh_adjoint_step(const mjhModel* m, const mjhData* d, const float* A, const 
  * the LDS of one workgroup gets NaN gradients. */
 int mjh_adjoint_vjp(const mjhModel* m, const mjhData* d, const float* qpos, const float* qvel, const float* lam_next,
                     const float* gadd, float* lam, float* grad_ctrl, mjhStream stream);
+/* The adjoint's tape (no counterpart in the reference): the Data fields
+ * mjh_adjoint_vjp reads, saved after a step so that the reverse pass needs
+ * no recomputation.  mjh_adjoint_tape_bytes gives one slot's size for (m,
+ * d) (the pools at full capacity); mjh_adjoint_tape_save copies the
+ * fields of the step d just took into a slot (device memory of that size;
+ * pools up to their counters); mjh_adjoint_vjp_taped is mjh_adjoint_vjp
+ * reading those fields from the slot instead of d (d gives the sizes). */
+int mjh_adjoint_tape_bytes(const mjhModel* m, const mjhData* d, unsigned long long* bytes);
+int mjh_adjoint_tape_save(const mjhModel* m, const mjhData* d, void* slot, mjhStream stream);
+int mjh_adjoint_vjp_taped(const mjhModel* m, const mjhData* d, const void* slot, const float* qpos, const float* qvel,
+                          const float* lam_next, const float* gadd, float* lam, float* grad_ctrl, mjhStream stream);
 /* lam [nworld, nx] as gradients with respect to d's raw state: g_qpos
  * [nworld, nq] (a rotation's tangent gradient g becomes 2 q (0, g),
  * orthogonal to q), g_qvel [nworld, nv], g_act [nworld, na]. */

@@ -219,6 +219,9 @@ def lib():
   L.mjh_adjoint_step.argtypes = [P, P, P, P, P, P, P, P, P, P, P]
   L.mjh_adjoint_state_grad.argtypes = [P, P, P, P, P, P, P]
   L.mjh_adjoint_vjp.argtypes = [P, P, P, P, P, P, P, P, P]
+  L.mjh_adjoint_tape_bytes.argtypes = [P, P, P]
+  L.mjh_adjoint_tape_save.argtypes = [P, P, P, P]
+  L.mjh_adjoint_vjp_taped.argtypes = [P, P, P, P, P, P, P, P, P, P]
   return L
 
 

@@ -19,6 +19,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 #include "adjoint.hpp"
 #include "stages.hpp"
@@ -220,6 +221,36 @@ __global__ void __launch_bounds__(64) k_adjoint_vjp(mjhModel m, mjhData d, const
   extern __shared__ float smem[];
   Team<kTeam> t(d.nworld);
   adjoint_vjp(m, d, qpos_t, qvel_t, lam_next, gadd, lam, grad_ctrl, rcap, smem, t);
+}
+// The adjoint's tape: copies of the Data fields the step adjoint reads, one
+// slot per step.  A segment is a field's words; a pool field copies only
+// its first count (device counter, clamped to the capacity) entries.
+struct TapeSeg {
+  const char* src;
+  unsigned long long dst;    // byte offset in the slot
+  unsigned long long bytes;  // capacity
+  const int* count;          // entries in use (pools), or null
+  unsigned int row;          // bytes per entry (pools)
+};
+constexpr int kTapeSegsMax = 40;
+struct TapeSegs {
+  TapeSeg s[kTapeSegsMax];
+  int n;
+};
+__global__ void __launch_bounds__(256) k_tape_copy(TapeSegs segs, char* slot) {
+  const TapeSeg& g = segs.s[blockIdx.y];
+  unsigned long long nb = g.bytes;
+  if (g.count) {
+    const long long c = *g.count;
+    const unsigned long long used = c > 0 ? (unsigned long long)c * g.row : 0ull;
+    if (used < nb) nb = used;
+  }
+  const unsigned int* src = reinterpret_cast<const unsigned int*>(g.src);
+  unsigned int* dst = reinterpret_cast<unsigned int*>(slot + g.dst);
+  const unsigned long long n4 = nb / 4;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 __global__ void __launch_bounds__(64) k_adjoint_state_grad(mjhModel m, mjhData d, const float* lam, float* g_qpos,
                                                            float* g_qvel, float* g_act) {
@@ -1196,28 +1227,120 @@ int mjh_adjoint_step(const mjhModel* m, const mjhData* d, const float* A, const 
                      g_act, lam_next, lam, grad_ctrl);
   return check_launch("adjoint_step");
 }
-int mjh_adjoint_vjp(const mjhModel* m, const mjhData* d, const float* qpos, const float* qvel, const float* lam_next,
-                    const float* gadd, float* lam, float* grad_ctrl, mjhStream stream) {
+}  // extern "C"
+
+namespace {
+
+// The slot layout: every field the adjoint reads, 256-byte aligned, pools
+// at full capacity (their indices stay valid).  Fills the segments to copy
+// (segs) and the Data whose fields point into the slot (out); returns the
+// slot's bytes.
+size_t tape_plan(const mjhModel& m, const mjhData& d, char* slot, mjhData* out, TapeSegs* segs) {
+  mjhData t = d;
+  size_t off = 0;
+  int k = 0;
+  auto seg = [&](auto& field, const void* src, size_t bytes, const int* count, unsigned row) {
+    off = (off + 255) & ~(size_t)255;
+    if (segs && k < kTapeSegsMax) segs->s[k] = TapeSeg{(const char*)src, off, bytes, count, row};
+    k++;
+    field = reinterpret_cast<std::remove_reference_t<decltype(field)>>(slot + off);
+    off += bytes;
+  };
+  const size_t W = d.nworld, nv = m.nv, nb = m.nbody, nu = m.nu, ng = m.ngeom, F = 4;
+  const size_t J = d.njmax, C = d.nconmax;
+  seg(t.qvel, d.qvel, W * nv * F, nullptr, 0);
+  seg(t.cdof, d.cdof, W * nv * 6 * F, nullptr, 0);
+  seg(t.cdof_dot, d.cdof_dot, W * nv * 6 * F, nullptr, 0);
+  seg(t.cinert, d.cinert, W * nb * 10 * F, nullptr, 0);
+  seg(t.subtree_com, d.subtree_com, W * nb * 3 * F, nullptr, 0);
+  seg(t.qMd, d.qMd, W * nv * nv * F, nullptr, 0);
+  seg(t.qacc, d.qacc, W * nv * F, nullptr, 0);
+  seg(t.qacc_integration, d.qacc_integration, W * nv * F, nullptr, 0);
+  seg(t.ctrl, d.ctrl, W * nu * F, nullptr, 0);
+  seg(t.actuator_moment, d.actuator_moment, W * nu * nv * F, nullptr, 0);
+  seg(t.actuator_length, d.actuator_length, W * nu * F, nullptr, 0);
+  seg(t.actuator_velocity, d.actuator_velocity, W * nu * F, nullptr, 0);
+  seg(t.geom_xpos, d.geom_xpos, W * ng * 3 * F, nullptr, 0);
+  seg(t.geom_xmat, d.geom_xmat, W * ng * 9 * F, nullptr, 0);
+  seg(t.world_efc, d.world_efc, W * 2 * F, nullptr, 0);
+  seg(t.world_con, d.world_con, W * 2 * F, nullptr, 0);
+  seg(t.efc_J, d.efc_J, J * nv * F, d.nefc, (unsigned)(nv * F));
+  seg(t.efc_D, d.efc_D, J * F, d.nefc, F);
+  seg(t.efc_force, d.efc_force, J * F, d.nefc, F);
+  seg(t.efc_pos, d.efc_pos, J * F, d.nefc, F);
+  seg(t.efc_margin, d.efc_margin, J * F, d.nefc, F);
+  seg(t.contact_geom, d.contact_geom, C * 2 * F, d.ncon, 2 * F);
+  seg(t.contact_dim, d.contact_dim, C * F, d.ncon, F);
+  seg(t.contact_pos, d.contact_pos, C * 3 * F, d.ncon, 3 * F);
+  seg(t.contact_frame, d.contact_frame, C * 9 * F, d.ncon, 9 * F);
+  seg(t.contact_friction, d.contact_friction, C * 5 * F, d.ncon, 5 * F);
+  seg(t.contact_dist, d.contact_dist, C * F, d.ncon, F);
+  seg(t.contact_includemargin, d.contact_includemargin, C * F, d.ncon, F);
+  seg(t.contact_solref, d.contact_solref, C * 2 * F, d.ncon, 2 * F);
+  seg(t.contact_solimp, d.contact_solimp, C * 5 * F, d.ncon, 5 * F);
+  if (segs) segs->n = k < kTapeSegsMax ? k : kTapeSegsMax;
+  if (out) *out = t;
+  return (off + 255) & ~(size_t)255;
+}
+
+int check_adjoint(const mjhModel* m, const mjhData* d) {
   if (!m || !d) return fail(MJH_ERR_INVALID, "null model or data");
-  if (!qpos || !qvel || !lam_next || !lam || (m->nu > 0 && !grad_ctrl))
-    return fail(MJH_ERR_INVALID, "adjoint_vjp: qpos, qvel, lam_next, lam (and grad_ctrl with actuators) must be set");
   if (m->na > 0) return fail(MJH_ERR_UNSUPPORTED, "adjoint_vjp: activation states (na > 0) are not differentiated");
   if (m->opt.integrator != mjhINT_EULER && m->opt.integrator != mjhINT_IMPLICITFAST)
     return fail(MJH_ERR_UNSUPPORTED, "adjoint_vjp: integrator must be Euler or implicitfast");
   if (m->actuator_affine_bias_gain && !(m->opt.disableflags & mjhDSBL_ACTUATION))
     return fail(MJH_ERR_UNSUPPORTED, "adjoint_vjp: affine actuator gains / biases are not differentiated");
-  if (d->nworld <= 0) return MJH_OK;
-  const int fixed = lds_adjoint_floats(m->nbody, m->nv, 0);
+  return MJH_OK;
+}
+
+int launch_vjp(const mjhModel& m, const mjhData& d, const float* qpos, const float* qvel, const float* lam_next,
+               const float* gadd, float* lam, float* grad_ctrl, hipStream_t s) {
+  if (d.nworld <= 0) return MJH_OK;
+  const int fixed = lds_adjoint_floats(m.nbody, m.nv, 0);
   const int cap = (int)(lds_cap() / sizeof(float));
   // rows per world the LDS holds: the pool's per-world share, at most what fits
-  int rcap = d->njmax / d->nworld + 1;
+  int rcap = d.njmax / d.nworld + 1;
   if (rcap > (cap - fixed) / 8) rcap = (cap - fixed) / 8;
   if (rcap < 4) return fail(MJH_ERR_CAPACITY, "adjoint_vjp: nv too large for the LDS of one workgroup");
-  const size_t bytes = (size_t)lds_adjoint_floats(m->nbody, m->nv, rcap) * sizeof(float);
+  const size_t bytes = (size_t)lds_adjoint_floats(m.nbody, m.nv, rcap) * sizeof(float);
   TRY(allow_lds((const void*)k_adjoint_vjp, bytes, "adjoint_vjp"));
-  hipLaunchKernelGGL(k_adjoint_vjp, dim3(d->nworld), dim3(64), bytes, (hipStream_t)stream, *m, *d, qpos, qvel,
-                     lam_next, gadd, lam, grad_ctrl, rcap);
+  hipLaunchKernelGGL(k_adjoint_vjp, dim3(d.nworld), dim3(64), bytes, s, m, d, qpos, qvel, lam_next, gadd, lam,
+                     grad_ctrl, rcap);
   return check_launch("adjoint_vjp");
+}
+
+}  // namespace
+
+extern "C" {
+
+int mjh_adjoint_tape_bytes(const mjhModel* m, const mjhData* d, unsigned long long* bytes) {
+  if (!m || !d || !bytes) return fail(MJH_ERR_INVALID, "adjoint_tape_bytes: null argument");
+  *bytes = tape_plan(*m, *d, nullptr, nullptr, nullptr);
+  return MJH_OK;
+}
+int mjh_adjoint_tape_save(const mjhModel* m, const mjhData* d, void* slot, mjhStream stream) {
+  if (!m || !d || !slot) return fail(MJH_ERR_INVALID, "adjoint_tape_save: null argument");
+  TapeSegs segs;
+  tape_plan(*m, *d, (char*)slot, nullptr, &segs);
+  if (segs.n >= kTapeSegsMax) return fail(MJH_ERR_CAPACITY, "adjoint_tape_save: too many fields");
+  hipLaunchKernelGGL(k_tape_copy, dim3(64, segs.n), dim3(256), 0, (hipStream_t)stream, segs, (char*)slot);
+  return check_launch("adjoint_tape_save");
+}
+int mjh_adjoint_vjp_taped(const mjhModel* m, const mjhData* d, const void* slot, const float* qpos, const float* qvel,
+                          const float* lam_next, const float* gadd, float* lam, float* grad_ctrl, mjhStream stream) {
+  TRY(check_adjoint(m, d));
+  if (!slot || !qpos || !qvel || !lam_next || !lam || (m->nu > 0 && !grad_ctrl))
+    return fail(MJH_ERR_INVALID, "adjoint_vjp_taped: slot, qpos, qvel, lam_next, lam (and grad_ctrl) must be set");
+  mjhData taped;
+  tape_plan(*m, *d, (char*)slot, &taped, nullptr);
+  return launch_vjp(*m, taped, qpos, qvel, lam_next, gadd, lam, grad_ctrl, (hipStream_t)stream);
+}
+int mjh_adjoint_vjp(const mjhModel* m, const mjhData* d, const float* qpos, const float* qvel, const float* lam_next,
+                    const float* gadd, float* lam, float* grad_ctrl, mjhStream stream) {
+  TRY(check_adjoint(m, d));
+  if (!qpos || !qvel || !lam_next || !lam || (m->nu > 0 && !grad_ctrl))
+    return fail(MJH_ERR_INVALID, "adjoint_vjp: qpos, qvel, lam_next, lam (and grad_ctrl with actuators) must be set");
+  return launch_vjp(*m, *d, qpos, qvel, lam_next, gadd, lam, grad_ctrl, (hipStream_t)stream);
 }
 int mjh_adjoint_state_grad(const mjhModel* m, const mjhData* d, const float* lam, float* g_qpos, float* g_qvel,
                            float* g_act, mjhStream stream) {

@@ -103,6 +103,26 @@ def vjp_step(m: Model, d: Data, qpos, qvel, lam_next, lam, grad_ctrl, gadd=None)
                _ptr(grad_ctrl))
 
 
+def tape_bytes(m: Model, d: Data) -> int:
+  """Bytes of one step's slot of the adjoint tape for (m, d)."""
+  n = ctypes.c_ulonglong(0)
+  _native.call("mjh_adjoint_tape_bytes", m, d, ctypes.byref(n))
+  return int(n.value)
+
+
+def tape_save(m: Model, d: Data, slot):
+  """Saves the fields of the step d just took that the adjoint reads into
+  `slot` (a uint8 device tensor of tape_bytes(m, d) bytes)."""
+  _native.call("mjh_adjoint_tape_save", m, d, _ptr(slot))
+
+
+def vjp_step_taped(m: Model, d: Data, slot, qpos, qvel, lam_next, lam, grad_ctrl, gadd=None):
+  """vjp_step with the step's fields read from a tape slot (tape_save)
+  instead of d."""
+  _native.call("mjh_adjoint_vjp_taped", m, d, _ptr(slot), _ptr(qpos), _ptr(qvel), _ptr(lam_next), _ptr(gadd),
+               _ptr(lam), _ptr(grad_ctrl))
+
+
 def check_analytic(mjm):
   """Raises NotImplementedError if the analytic adjoint does not cover the
   model (mjh_adjoint_vjp: Euler / implicitfast, free / hinge / slide joints,
@@ -147,17 +167,20 @@ class Rollout:
   (zeros by default).
 
   adjoint="analytic" (the default) runs the reverse pass with the
-  hand-written step adjoint (mjh_adjoint_vjp): each step of the backward
-  pass recomputes that step's forward quantities from its saved input
-  state (one fused step) and takes one vector-Jacobian product.
+  hand-written step adjoint (mjh_adjoint_vjp), one vector-Jacobian product
+  per step.  The fields it reads are taped during the forward pass
+  (tape_save: tape_bytes(m, d) per step, the pools at full capacity) when
+  nstep slots fit in `tape_budget` bytes; otherwise, or with tape=False,
+  each backward step recomputes its forward quantities from its saved input
+  state with one fused step.  checkpoint_every=k (recomputing only) keeps
+  the input state of every k-th step (memory / k) and recomputes each
+  segment of k steps from its checkpoint (one more step per step).
   adjoint="fd" uses finite-difference transition Jacobians instead
-  (TransitionFD; any model the step runs).  checkpoint_every=k keeps the
-  input state of every k-th step only (memory / k) and recomputes each
-  segment of k steps from its checkpoint in the backward pass (one more
-  step per step)."""
+  (TransitionFD; any model the step runs)."""
 
   def __init__(self, mjm, m: Model, nworld: int, nstep: int, nconmax: int = -1, njmax: int = -1, eps: float = 4e-3,
-               centered: bool = True, device=None, adjoint: str = "analytic", checkpoint_every: int = 1):
+               centered: bool = True, device=None, adjoint: str = "analytic", checkpoint_every: int = 1,
+               tape=None, tape_budget: int = 8 << 30):
     if adjoint not in ("analytic", "fd"):
       raise ValueError(f"adjoint must be 'analytic' or 'fd', got {adjoint!r}")
     if adjoint == "analytic":
@@ -182,6 +205,11 @@ class Rollout:
     self.lam = z(W, nx_of(mjm))
     self.lam_next = z(W, nx_of(mjm))
     self.lam_g = z(W, nx_of(mjm))
+    self.tape = None
+    if adjoint == "analytic" and tape is not False and self.every == 1:
+      nb = tape_bytes(m, self.d)
+      if tape or T * nb <= tape_budget:
+        self.tape = torch.empty((T, nb), dtype=torch.uint8, device=dev)
 
   def _save(self, buf, i):
     d = self.d
@@ -197,8 +225,10 @@ class Rollout:
     d.act.copy_(buf[2][i])
     d.qacc_warmstart.copy_(buf[3][i])
 
-  def forward(self, qpos0, qvel0, ctrl, act0=None, warmstart0=None):
-    """Runs the rollout without recording anything for autograd."""
+  def forward(self, qpos0, qvel0, ctrl, act0=None, warmstart0=None, record: bool = True):
+    """Runs the rollout without recording anything for autograd; with
+    record=False the tape is not written either (the states and checkpoints
+    are), so a later backward() is invalid."""
     d, T = self.d, self.nstep
     if tuple(ctrl.shape) != (T, self.nworld, self.nu):
       raise ValueError(f"ctrl must be [{T}, {self.nworld}, {self.nu}], got {tuple(ctrl.shape)}")
@@ -214,6 +244,8 @@ class Rollout:
         self._save(ck, t // self.every)
       d.ctrl.copy_(ctrl[t])
       api.step_fused(self.m, d)
+      if self.tape is not None and record:
+        tape_save(self.m, d, self.tape[t])
       qpos[t].copy_(d.qpos)
       qvel[t].copy_(d.qvel)
     return qpos, qvel
@@ -230,7 +262,18 @@ class Rollout:
     # lam_T = g(x_T), at x_T's qpos
     d.qpos.copy_(qpos[T - 1])
     adjoint_step(m, d, None, None, self.lam_next, g_qpos=g_qpos[T - 1], g_qvel=g_qvel[T - 1])
-    for s0 in range(((T - 1) // K) * K, -1, -K):
+    if self.tape is not None:
+      # every step's input state and fields are kept: no recomputation
+      for t in range(T - 1, -1, -1):
+        gadd = None
+        if t > 0:
+          d.qpos.copy_(self.c_qpos[t])
+          adjoint_step(m, d, None, None, self.lam_g, g_qpos=g_qpos[t - 1], g_qvel=g_qvel[t - 1])
+          gadd = self.lam_g
+        vjp_step_taped(m, d, self.tape[t], self.c_qpos[t], self.c_qvel[t], self.lam_next, self.lam, grad_ctrl[t],
+                       gadd)
+        self.lam, self.lam_next = self.lam_next, self.lam
+    for s0 in (range(((T - 1) // K) * K, -1, -K) if self.tape is None else ()):
       s1 = min(s0 + K, T)
       # the segment's input states, recomputed from its checkpoint
       self._restore(ck, s0 // K)

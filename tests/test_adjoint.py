@@ -196,15 +196,18 @@ def test_analytic_rollout_in_contact_reported():
   assert eu < 1e-3 and ev < 1e-3, (eu, ev)
 
 
-def test_checkpointed_rollout_gradients_are_identical():
-  """checkpoint_every=4 recomputes each segment from its checkpoint: the
-  steps replay bit for bit, so the gradients equal those of a rollout that
-  keeps every step's input."""
+def test_taped_and_recomputed_rollout_gradients_are_identical():
+  """The taped reverse pass (the default), recomputation from every step's
+  input (tape=False) and from a checkpoint every 4 steps give the same
+  gradients bit for bit: the steps replay exactly, and the adjoint reads
+  the same per-world fields."""
   mjm = models.humanoid(iterations=10, ls_iterations=20)
   n, T = 4, 10
   q0, v0 = models.humanoid_rollout_state(mjm, n, 3)
   ctrl = np.random.default_rng(5).uniform(-1, 1, (T, n, mjm.nu))
   a = _rollout_grads(mjm, q0, v0, ctrl)
-  b = _rollout_grads(mjm, q0, v0, ctrl, checkpoint_every=4)
-  for x, y in zip(a, b):
+  b = _rollout_grads(mjm, q0, v0, ctrl, tape=False)
+  c = _rollout_grads(mjm, q0, v0, ctrl, checkpoint_every=4)
+  for x, y, z in zip(a, b, c):
     assert np.array_equal(x, y)
+    assert np.array_equal(x, z)
