@@ -119,6 +119,67 @@ MJH_DEV S6 inert_moved(const float* I, const S6& s, const S6& x) {
   return s6_sub(motion_cross_force(s, inert_vec(I, x)), inert_vec(I, motion_cross(s, x)));
 }
 
+// closest_segment_point and its tangent (dA, dB, dPT: tangents of a, b, pt)
+MJH_DEV V3 csp_tangent(V3 a, V3 b, V3 pt, V3 dA, V3 dB, V3 dPT, V3* dout) {
+  const V3 ab = b - a, dab = dB - dA;
+  const float den = dot(ab, ab) + 1e-6f;
+  const float tt = dot(pt - a, ab) / den;
+  const float dtt = (dot(dPT - dA, ab) + dot(pt - a, dab)) / den - tt * 2.0f * dot(ab, dab) / den;
+  const float tc = fminf(fmaxf(tt, 0.0f), 1.0f);
+  const float dtc = (tt > 0.0f && tt < 1.0f) ? dtt : 0.0f;
+  *dout = dA + dtc * ab + tc * dab;
+  return a + tc * ab;
+}
+// closest_segment_to_segment_points (collision.hpp) and the tangent of its
+// two points for tangents dA0.. of the segment ends: the same operations,
+// clamps and choice, differentiated
+MJH_DEV void segseg_tangent(V3 a0, V3 a1, V3 b0, V3 b1, V3 dA0, V3 dA1, V3 dB0, V3 dB1, V3* ra, V3* rb, V3* dra,
+                            V3* drb) {
+  V3 da = a1 - a0, db = b1 - b0, dda = dA1 - dA0, ddb = dB1 - dB0;
+  const float la = length(da), lb = length(db);
+  if (la != 0.0f) {
+    da = da * (1.0f / la);
+    dda = (dda - da * dot(da, dda)) * (1.0f / la);
+  }
+  if (lb != 0.0f) {
+    db = db * (1.0f / lb);
+    ddb = (ddb - db * dot(db, ddb)) * (1.0f / lb);
+  }
+  const float dla = la != 0.0f ? dot(da, dA1 - dA0) : 0.0f, dlb = lb != 0.0f ? dot(db, dB1 - dB0) : 0.0f;
+  const float ha = la * 0.5f, hb = lb * 0.5f, dha = 0.5f * dla, dhb = 0.5f * dlb;
+  const V3 am = a0 + da * ha, bm = b0 + db * hb;
+  const V3 dam = dA0 + dda * ha + da * dha, dbm = dB0 + ddb * hb + db * dhb;
+  const V3 tr = am - bm, dtr = dam - dbm;
+  const float dadb = dot(da, db), datr = dot(da, tr), dbtr = dot(db, tr);
+  const float ddadb = dot(dda, db) + dot(da, ddb), ddatr = dot(dda, tr) + dot(da, dtr);
+  const float ddbtr = dot(ddb, tr) + dot(db, dtr);
+  const float denom = 1.0f - dadb * dadb, ddenom = -2.0f * dadb * ddadb;
+  const float num = -datr + dadb * dbtr, dnum = -ddatr + ddadb * dbtr + dadb * ddbtr;
+  const float den = denom + 1e-6f;
+  const float ota = num / den, dota = (dnum * den - num * ddenom) / (den * den);
+  const float otb = dbtr + ota * dadb, dotb = ddbtr + dota * dadb + ota * ddadb;
+  const float ta = fminf(fmaxf(ota, -ha), ha), tb = fminf(fmaxf(otb, -hb), hb);
+  const float dta = ota < -ha ? -dha : (ota > ha ? dha : dota);
+  const float dtb = otb < -hb ? -dhb : (otb > hb ? dhb : dotb);
+  const V3 ba = am + da * ta, bb = bm + db * tb;
+  const V3 dba = dam + dda * ta + da * dta, dbb = dbm + ddb * tb + db * dtb;
+  V3 dna, dnb;
+  const V3 na = csp_tangent(a0, a1, bb, dA0, dA1, dbb, &dna);
+  const V3 nb = csp_tangent(b0, b1, ba, dB0, dB1, dba, &dnb);
+  const float d1 = dot(bb - na, bb - na), d2 = dot(ba - nb, ba - nb);
+  if (d1 < d2) {
+    *ra = na;
+    *rb = bb;
+    *dra = dna;
+    *drb = dbb;
+  } else {
+    *ra = ba;
+    *rb = nb;
+    *dra = dba;
+    *drb = dnb;
+  }
+}
+
 // One world per workgroup of 64 lanes.  d holds the step just taken from
 // (qpos_t, qvel_t): the forward quantities at that state and qvel' in
 // d.qvel.  lam_next [nworld, nx] is lam'; lam [nworld, nx] gets
@@ -350,7 +411,7 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
   sync();
   const bool refsafe = !(dis & mjhDSBL_REFSAFE);
   for (int i = t.lane; i < n; i += TEAM) {
-    float cq = 0.0f, cvr = 0.0f, al = 0.0f, be = 0.0f, ga = 0.0f;
+    float cq = 0.0f, cvr = 0.0f, al = 0.0f, be = 0.0f, ga = 0.0f, cd = 0.0f;
     const int src = rsrc[i];
     if (fr[i] > 0.0f && src != 0x7fffffff) {
       const float* solref;
@@ -376,8 +437,18 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
         al = f;
         be = -D * g;
         ga = -D * g * b;
+        // sphere / capsule pairs: the distance's derivative comes from the
+        // narrowphase's own closest points (contact loop below), not the
+        // normal row
+        const int c = src >> 2;
+        const int t1 = m.geom_type[d.contact_geom[2 * c]], t2 = m.geom_type[d.contact_geom[2 * c + 1]];
+        if ((t1 == mjhGEOM_SPHERE || t1 == mjhGEOM_CAPSULE) && (t2 == mjhGEOM_SPHERE || t2 == mjhGEOM_CAPSULE)) {
+          cd = cq;
+          cq = 0.0f;
+        }
       }
     }
+    rlf[i] = cd;  // (the row multipliers are no longer needed)
     rcq[i] = cq;
     rcv[i] = cvr;
     ral[i] = al;
@@ -414,7 +485,10 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
     const float* len = WPTR(d.actuator_length, w, nu);
     const float* vel = WPTR(d.actuator_velocity, w, nu);
     const bool noclamp = dis & mjhDSBL_CLAMPCTRL;
-    for (int a = t.lane; a < nu; a += TEAM) {
+    // d qfrc_actuator / d ctrl_a = gain_a (inside ctrlrange) times the
+    // moment entries the step applies: all of them for sparse models, the
+    // dense tiles' (the reference's tile loop) otherwise
+    auto act = [&](int a, int onv, int tnv) {
       bool inr = true;
       if (m.actuator_ctrllimited[a] && !noclamp) {
         const float c = ctrl[a];
@@ -424,10 +498,20 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
       if (inr) {
         const float* g = m.actuator_gainprm + 10 * a;
         const float gain = g[0] + g[1] * len[a] + g[2] * vel[a];
-        for (int j = 0; j < nv; j++) s += moment[(size_t)a * nv + j] * wv[j];
+        for (int j = onv; j < onv + tnv; j++) s += moment[(size_t)a * nv + j] * wv[j];
         s *= gain;
       }
       grad_ctrl[(size_t)w * nu + a] = s;
+    };
+    if (m.opt.is_sparse) {
+      for (int a = t.lane; a < nu; a += TEAM) act(a, 0, nv);
+    } else {
+      for (int a = t.lane; a < nu; a += TEAM) grad_ctrl[(size_t)w * nu + a] = 0.0f;
+      sync();
+      for_each_act_tile(m, m.nact_group, [&](int onv, int onu, int tnv, int tnu) {
+        if (tnu == 0 || tnv == 0) return;
+        for (int a = t.lane; a < tnu; a += TEAM) act(onu + a, onv, tnv);
+      });
     }
   } else {
     for (int a = t.lane; a < nu; a += TEAM) grad_ctrl[(size_t)w * nu + a] = 0.0f;
@@ -627,53 +711,37 @@ MJH_DEV void adjoint_vjp(const mjhModel& m, const mjhData& d, const float* qpos_
           }
         }
       } else if (round) {
-        // sphere / capsule pairs (sphere_sphere on the closest points of the
-        // core segments): the closest points slide along the segments as
-        // the optimality conditions (dir . a_i = 0 for a free parameter)
-        // require; n = dir / |dir|, p = P1 + (r1 + dist / 2) n, frame =
-        // make_frame(n)
+        // sphere / capsule pairs: sphere_sphere on the closest points of the
+        // core segments (closest_segment_to_segment_points differentiated
+        // as computed, clamps and choice included); n = dir / |dir|,
+        // p = P1 + (r1 + dist / 2) n, frame = make_frame(n)
         const float* gxp = d.geom_xpos + (size_t)w * m.ngeom * 3;
         const float* gxm = d.geom_xmat + (size_t)w * m.ngeom * 9;
-        const float r1 = m.geom_size[3 * g1], r2 = m.geom_size[3 * g2];
+        const float r1 = m.geom_size[3 * g1];
         const float dist = d.contact_dist[c];
-        const float L = dist + r1 + r2;
+        const float L = dist + r1 + m.geom_size[3 * g2];
         if (L > 1e-6f) {
-          const V3 P1 = p - nrm * (r1 + 0.5f * dist);
-          const V3 dirv = nrm * L;
-          const V3 P2 = P1 + dirv;
-          const V3 a1 = V3{gxm[9 * g1 + 2], gxm[9 * g1 + 5], gxm[9 * g1 + 8]};
-          const V3 a2 = V3{gxm[9 * g2 + 2], gxm[9 * g2 + 5], gxm[9 * g2 + 8]};
-          const float h1 = ty1 == mjhGEOM_CAPSULE ? m.geom_size[3 * g1 + 1] : 0.0f;
-          const float h2 = ty2 == mjhGEOM_CAPSULE ? m.geom_size[3 * g2 + 1] : 0.0f;
-          const float cc = dot(a1, a2);
-          bool cl1 = h1 == 0.0f || fabsf(dot(P1 - ld3(gxp + 3 * g1), a1)) >= h1 * (1.0f - 1e-4f);
-          const bool cl2 = h2 == 0.0f || fabsf(dot(P2 - ld3(gxp + 3 * g2), a2)) >= h2 * (1.0f - 1e-4f);
-          if (!cl1 && !cl2 && 1.0f - cc * cc < 1e-4f) cl1 = true;  // parallel segments
-          V3 dP1 = V3{0.f, 0.f, 0.f}, dP2 = V3{0.f, 0.f, 0.f}, w1 = V3{0.f, 0.f, 0.f}, w2 = V3{0.f, 0.f, 0.f};
-          if (b1 != 0 && m.body_dof_tree[b1 * nv + k]) {
-            dP1 = s6_lin(S) + cross(s6_rot(S), P1 - ld3(sc + 3 * m.body_rootid[b1]));
-            w1 = s6_rot(S);
+          auto matv = [&](int b, V3 P) {
+            if (b == 0 || !m.body_dof_tree[b * nv + k]) return V3{0.f, 0.f, 0.f};
+            return s6_lin(S) + cross(s6_rot(S), P - ld3(sc + 3 * m.body_rootid[b]));
+          };
+          const V3 c1 = ld3(gxp + 3 * g1), c2 = ld3(gxp + 3 * g2);
+          V3 P1, P2, dP1, dP2;
+          if (ty1 == mjhGEOM_CAPSULE && ty2 == mjhGEOM_CAPSULE) {
+            const V3 s1 = V3{gxm[9 * g1 + 2], gxm[9 * g1 + 5], gxm[9 * g1 + 8]} * m.geom_size[3 * g1 + 1];
+            const V3 s2 = V3{gxm[9 * g2 + 2], gxm[9 * g2 + 5], gxm[9 * g2 + 8]} * m.geom_size[3 * g2 + 1];
+            segseg_tangent(c1 - s1, c1 + s1, c2 - s2, c2 + s2, matv(b1, c1 - s1), matv(b1, c1 + s1),
+                           matv(b2, c2 - s2), matv(b2, c2 + s2), &P1, &P2, &dP1, &dP2);
+          } else {
+            dP1 = matv(b1, c1);
+            dP2 = matv(b2, c2);
           }
-          if (b2 != 0 && m.body_dof_tree[b2 * nv + k]) {
-            dP2 = s6_lin(S) + cross(s6_rot(S), P2 - ld3(sc + 3 * m.body_rootid[b2]));
-            w2 = s6_rot(S);
-          }
-          const V3 dm = dP2 - dP1;
-          const float rr1 = dot(dm, a1) + dot(dirv, cross(w1, a1));
-          const float rr2 = dot(dm, a2) + dot(dirv, cross(w2, a2));
-          float ds1 = 0.0f, ds2 = 0.0f;
-          if (!cl1 && !cl2) {
-            ds2 = (cc * rr1 - rr2) / (1.0f - cc * cc);
-            ds1 = rr1 + cc * ds2;
-          } else if (cl1 && !cl2) {
-            ds2 = -rr2;
-          } else if (!cl1 && cl2) {
-            ds1 = rr1;
-          }
-          const V3 ddir = dm + a2 * ds2 - a1 * ds1;
+          const V3 ddir = dP2 - dP1;
           const V3 dn = (ddir - nrm * dot(nrm, ddir)) * (1.0f / L);
           const float ddist = dot(nrm, ddir);
-          dp = dP1 + a1 * ds1 + dn * (r1 + 0.5f * dist) + nrm * (0.5f * ddist);
+          dp = dP1 + dn * (r1 + 0.5f * dist) + nrm * (0.5f * ddist);
+          // the row's distance terms (aref, D) through this distance
+          accc += rlf[i] * ddist;
           const V3 ev = (-0.5f < nrm.y && nrm.y < 0.5f) ? V3{0.f, 1.f, 0.f} : V3{0.f, 0.f, 1.f};
           const V3 braw = ev - nrm * dot(nrm, ev);
           const float bl = length(braw);

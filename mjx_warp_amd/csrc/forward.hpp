@@ -29,6 +29,15 @@ MJH_DEV void actuator_velocity(const mjhModel& m, const mjhData& d, const Team<T
   const float* moment = WPTR(d.actuator_moment, w, nu * nv);
   const float* qvel = WPTR(d.qvel, w, nv);
   float* vel = WPTR(d.actuator_velocity, w, nu);
+  // sparse models: moment * qvel over all actuators and dofs (forward.py:427-438)
+  if (m.opt.is_sparse) {
+    for (int a = t.lane; a < nu; a += TEAM) {
+      float s = 0.0f;
+      for (int j = 0; j < nv; j++) s += moment[(size_t)a * nv + j] * qvel[j];
+      vel[a] = s;
+    }
+    return;
+  }
   for_each_act_tile(m, m.nact_group, [&](int onv, int onu, int tnv, int tnu) {
     if (tnu == 0 || tnv == 0) return;
     for (int a = t.lane; a < tnu; a += TEAM) {
@@ -90,14 +99,23 @@ MJH_DEV void fwd_actuation(const mjhModel& m, const mjhData& d, const Team<TEAM>
   sync();
   if (t.valid) {
     const float* moment = WPTR(d.actuator_moment, w, nu * nv);
-    for_each_act_tile(m, m.nact_group, [&](int onv, int onu, int tnv, int tnu) {
-      if (tnu == 0 || tnv == 0) return;
-      for (int j = t.lane; j < tnv; j += TEAM) {
+    if (m.opt.is_sparse) {
+      // sparse models: moment' force over all actuators and dofs (forward.py:571-578)
+      for (int j = t.lane; j < nv; j += TEAM) {
         float s = 0.0f;
-        for (int a = 0; a < tnu; a++) s += moment[(onu + a) * nv + onv + j] * force[onu + a];
-        qfrc[onv + j] = s;
+        for (int a = 0; a < nu; a++) s += moment[(size_t)a * nv + j] * force[a];
+        qfrc[j] = s;
       }
-    });
+    } else {
+      for_each_act_tile(m, m.nact_group, [&](int onv, int onu, int tnv, int tnu) {
+        if (tnu == 0 || tnv == 0) return;
+        for (int j = t.lane; j < tnv; j += TEAM) {
+          float s = 0.0f;
+          for (int a = 0; a < tnu; a++) s += moment[(onu + a) * nv + onv + j] * force[onu + a];
+          qfrc[onv + j] = s;
+        }
+      });
+    }
   }
   sync();
   if (t.valid)

@@ -639,6 +639,14 @@ void fwd_velocity(const Model<T>& m, const Data<T>& d, int w) {
   const T* mom = d.actuator_moment + (size_t)w * nu * nv;
   const T* qvel = d.qvel + (size_t)w * nv;
   T* vel = d.actuator_velocity + (size_t)w * nu;
+  // sparse models: moment * qvel over all actuators and dofs (forward.py:427-438)
+  if (m.opt.is_sparse) {
+    for (int a = 0; a < nu; a++) {
+      T s = 0;
+      for (int j = 0; j < nv; j++) s += mom[a * nv + j] * qvel[j];
+      vel[a] = s;
+    }
+  } else
   each_act_tile(m, m.nact_group, [&](int onv, int onu, int tnv, int tnu) {
     if (tnu == 0 || tnv == 0) return;
     for (int a = 0; a < tnu; a++) {
@@ -675,6 +683,14 @@ void fwd_actuation(const Model<T>& m, const Data<T>& d, int w) {
     force[a] = gain * c + bias;  // forcerange is not applied (forward.py:539-541)
   }
   const T* mom = d.actuator_moment + (size_t)w * nu * nv;
+  // sparse models: moment' force over all actuators and dofs (forward.py:571-578)
+  if (m.opt.is_sparse) {
+    for (int j = 0; j < nv; j++) {
+      T s = 0;
+      for (int a = 0; a < nu; a++) s += mom[a * nv + j] * force[a];
+      qfrc[j] = s;
+    }
+  } else
   each_act_tile(m, m.nact_group, [&](int onv, int onu, int tnv, int tnu) {
     if (tnu == 0 || tnv == 0) return;
     for (int j = 0; j < tnv; j++) {

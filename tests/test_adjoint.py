@@ -211,3 +211,72 @@ def test_taped_and_recomputed_rollout_gradients_are_identical():
   for x, y, z in zip(a, b, c):
     assert np.array_equal(x, y)
     assert np.array_equal(x, z)
+
+
+def test_vjp_chain_euler_matches_oracle_fd():
+  """The 3-link hinge chain (Euler, joint limits, capsule links against the
+  floor plane), 8 worlds with the chain hanging into the floor, to 1e-3.
+  The adjoint differentiates the constraint optimum, whichever solver
+  reached it; the reference is taken with Newton: the chain's CG at 200
+  iterations leaves a world whose finite differences at eps 1e-6 and 1e-5
+  differ by 1.2e-2 (truncation, not the state), while Newton's agree with
+  the analytic derivative to 1e-8 in FP64."""
+  mjm = models.chain(3)
+  mjm.opt.solver = int(mjw.SolverType.NEWTON)
+  mjm.opt.iterations = 100
+  mjm.opt.tolerance = 1e-15
+  mjm.opt.ls_tolerance = 1e-15
+  n = 8
+  q0, v0 = models.chain_initial_state(n, seed=4)
+  q0 *= 0.3
+  u = np.zeros((n, 0))
+  errs = []
+  A, B = _oracle_jac(mjm, q0, v0, u)
+  rng = np.random.default_rng(21)
+  lam = rng.normal(size=(n, 2 * mjm.nv))
+  got, _, rows = _device_vjp(mjm, q0, v0, u, lam)
+  ref = np.einsum("wi,wij->wj", lam, A)
+  assert rows.max() > 0, "the case needs constraint rows"
+  e = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+  print(f"chain (rows {rows.tolist()}): worst rel. error {e.max():.2e}")
+  assert e.max() < 1e-3, e
+
+
+def test_vjp_large_model_lds_path_matches_oracle_fd():
+  """n_humanoids.xml (three attached humanoids, nv 81: past the register
+  Cholesky's 32 rows, so both solves go through the LDS factor; a sparse
+  model, so every actuator acts), lowered into the floor with 82 and 119
+  constraint rows: the state columns to 5e-3, the control columns to 1e-3.
+  The analytic derivative is exact here in FP64 (2e-8 against the FP64
+  oracle's differences); the device's error is that of its FP32 forward
+  state: with 119 stiff rows the FP32 optimum (forces, active set) sits
+  ~1e-3 from the FP64 one (measured 1.7e-3), while the device product
+  agrees with the same formula in FP64 on the device's own forward fields
+  to 1.6e-4."""
+  from mjx_warp_amd import mjcf
+
+  mjm = mjcf.from_xml_string((models.ASSETS / "n_humanoids.xml").read_text(), base_dir=str(models.ASSETS))
+  mjm.opt.iterations = 100
+  mjm.opt.ls_iterations = 50
+  mjm.opt.tolerance = 1e-15
+  mjm.opt.ls_tolerance = 1e-15
+  derivative.check_analytic(mjm)
+  n = 2
+  rng = np.random.default_rng(41)
+  q0 = np.tile(mjm.qpos0, (n, 1))
+  for k in range(3):
+    q0[:, 28 * k + 2] -= 0.05
+    q0[:, 28 * k + 7:28 * (k + 1)] += rng.normal(0, 0.1, (n, 21))
+  v0 = rng.normal(0, 0.3, (n, mjm.nv))
+  u = rng.uniform(-1, 1, (n, mjm.nu))
+  step = helpers.oracle_stepper(mjm, mjw.put_model(mjm, device="cpu"), nthreads=16)
+  A, B = od.transition_fd(step, mjm, q0, v0, np.zeros((n, 0)), u, np.zeros((n, mjm.nv)), 1e-6)
+  lam = rng.normal(size=(n, 2 * mjm.nv))
+  got, gu, rows = _device_vjp(mjm, q0, v0, u, lam)
+  ref = np.einsum("wi,wij->wj", lam, A)
+  refu = np.einsum("wi,wij->wj", lam, B)
+  assert rows.min() > 0, "the case needs constraint rows"
+  e = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+  eu = np.linalg.norm(gu - refu, axis=1) / np.linalg.norm(refu, axis=1)
+  print(f"n_humanoids (rows {rows.tolist()}): state columns {e.max():.2e}, ctrl {eu.max():.2e}")
+  assert e.max() < 5e-3 and eu.max() < 1e-3, (e, eu)

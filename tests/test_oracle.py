@@ -746,3 +746,25 @@ def test_ball_and_free_joint_kinematics_closed_form():
     np.testing.assert_allclose(hd["xpos"][w, 2], x[w] + Rf.apply(o), atol=1e-12)
     np.testing.assert_allclose(hd["xipos"][w, 2], x[w] + Rf.apply(o + Rb.apply(c)), atol=1e-12)
     np.testing.assert_allclose(hd["xanchor"][w, 1], x[w] + Rf.apply(o), atol=1e-12)
+
+
+def test_actuator_forces_cover_every_tree():
+  """Actuator forces reach every kinematic tree.  n_humanoids.xml is a
+  sparse model (nv 81), for which the reference sums moment' force and
+  moment * qvel over all actuators and dofs (forward.py:427-438, 571-578);
+  its dense tile loop ends the last tile group at the number of groups
+  (forward.py:629-633), so the 3 trees of one tile size would get only the
+  first tree's forces.  Both quantities must equal the full products."""
+  mjm = mjcf.from_xml_string((models.ASSETS / "n_humanoids.xml").read_text(), base_dir=str(models.ASSETS))
+  _, m, d, hm, hd = helpers.load(mjm, nworld=2)
+  assert m.opt.is_sparse
+  rng = np.random.default_rng(3)
+  hd["qvel"][:] = rng.normal(0, 0.5, hd["qvel"].shape)
+  hd["ctrl"][:] = rng.uniform(-1, 1, hd["ctrl"].shape)
+  oracle.run("forward", hm, hd)
+  for w in range(2):
+    mom = hd["actuator_moment"][w].reshape(mjm.nu, mjm.nv)
+    np.testing.assert_allclose(hd["qfrc_actuator"][w], mom.T @ hd["actuator_force"][w], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(hd["actuator_velocity"][w], mom @ hd["qvel"][w], rtol=1e-12, atol=1e-12)
+    # every humanoid's 21 motors move its own tree
+    assert all(np.abs(hd["qfrc_actuator"][w][27 * k + 6:27 * (k + 1)]).max() > 0 for k in range(3))
