@@ -353,6 +353,18 @@ def time_heterogeneous(args, nworld: int = 16384, steps: int = 50, graph_steps: 
   torch.cuda.synchronize()
   out["grouped_graph"] = nworld * reps * graph_steps / (time.perf_counter() - t0)
   ms.close()
+  # the same grouped step with work-queue launches
+  mq = mjw.MultiStep(groups, queue=True)
+  restore()
+  for _ in range(3):
+    mq.step()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(steps):
+    mq.step()
+  torch.cuda.synchronize()
+  out["grouped_queue"] = nworld * steps / (time.perf_counter() - t0)
+  mq.close()
   t0 = time.perf_counter()
   restore()
   tuned = mjw.MultiStep(groups, autotune=True)
@@ -369,7 +381,7 @@ def time_heterogeneous(args, nworld: int = 16384, steps: int = 50, graph_steps: 
   teams = {}
   for t in tuned.tunes or []:
     teams[f"team{t[0]}_rj{t[2]}"] = teams.get(f"team{t[0]}_rj{t[2]}", 0) + 1
-  out["autotuned_choice"] = {"mode": tuned.mode, "buckets": tuned.buckets, "per_model": teams}
+  out["autotuned_choice"] = {"mode": tuned.mode, "buckets": tuned.buckets, "queue": tuned.queue, "per_model": teams}
   out["value"] = out["autotuned"]
   out["unit"] = "world-steps/s"
   out["finite"] = bool(all(torch.isfinite(d.qpos).all().item() for _, d in groups))
@@ -431,8 +443,8 @@ def time_heterogeneous_stated(args, nworld: int = 16384, steps: int = 100):
   out = {"nworld": nworld, "models": len(groups), "steps": steps, "setup_s": t_setup,
          "nv_range": [min(int(m.nv) for m, _ in groups), max(int(m.nv) for m, _ in groups)]}
   best = None
-  for buckets in (False, True):
-    ms = mjw.MultiStep(groups, buckets=buckets)
+  for buckets, queue in ((False, False), (True, False), (True, True)):
+    ms = mjw.MultiStep(groups, buckets=buckets, queue=queue)
     restore()
     ms.step()
     torch.cuda.synchronize()
@@ -442,11 +454,11 @@ def time_heterogeneous_stated(args, nworld: int = 16384, steps: int = 100):
       ms.step()
     torch.cuda.synchronize()
     rate = nworld * 10 / (time.perf_counter() - t1)
-    out["grouped_buckets" if buckets else "grouped"] = rate
+    out["grouped" + ("_buckets" if buckets else "") + ("_queue" if queue else "")] = rate
     if best is None or rate > best[0]:
-      best = (rate, buckets)
+      best = (rate, buckets, queue)
     ms.close()
-  ms = mjw.MultiStep(groups, buckets=best[1])
+  ms = mjw.MultiStep(groups, buckets=best[1], queue=best[2])
   restore()
   g = torch.cuda.CUDAGraph()
   side = torch.cuda.Stream()
@@ -462,6 +474,7 @@ def time_heterogeneous_stated(args, nworld: int = 16384, steps: int = 100):
   torch.cuda.synchronize()
   out["graph_rollout"] = nworld * steps / (time.perf_counter() - t1)
   out["buckets"] = best[1]
+  out["queue"] = best[2]
   out["value"] = out["graph_rollout"]
   out["unit"] = "world-steps/s"
   out["finite"] = bool(all(torch.isfinite(d.qpos).all().item() for _, d in groups))
@@ -665,14 +678,16 @@ def main():
   if not args.no_hetero:
     het = time_heterogeneous(args)
     log(f"heterogeneous: grouped {het['grouped']:.4g}, per model {het['per_group']:.4g}, "
-        f"grouped graph {het['grouped_graph']:.4g}, autotuned {het['autotuned']:.4g} world-steps/s "
+        f"grouped graph {het['grouped_graph']:.4g}, grouped work queue {het['grouped_queue']:.4g}, "
+        f"autotuned {het['autotuned']:.4g} world-steps/s "
         f"({het['autotuned_choice']}, tuning {het['autotune_s']:.1f} s)")
   het_stated = None
   if not args.no_hetero_stated:
     het_stated = time_heterogeneous_stated(args)
     log(f"heterogeneous at stated size: {het_stated['models']} models, nv {het_stated['nv_range']}, "
         f"{het_stated['rows_per_world']:.1f} rows per world: grouped {het_stated['grouped']:.4g}, "
-        f"buckets {het_stated['grouped_buckets']:.4g}, {het_stated['steps']}-step graph rollout "
+        f"buckets {het_stated['grouped_buckets']:.4g}, buckets + work queue {het_stated['grouped_buckets_queue']:.4g}, "
+        f"{het_stated['steps']}-step graph rollout "
         f"{het_stated['graph_rollout']:.4g} world-steps/s")
   adj = None
   if not args.no_adjoint:

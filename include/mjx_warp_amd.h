@@ -499,6 +499,11 @@ typedef struct mjhGroupTune {
  * group's launches), each bucket's launches sized for its own groups; without
  * it every launch takes the largest group's LDS. */
 #define MJH_MULTI_BUCKETS 1
+/* Work-queue launches: each stage launch of the plan is a grid of a few
+ * workgroups per SIMD that take worlds from a counter in device memory
+ * (largest models first) until none are left, instead of one workgroup per
+ * world (or pair of worlds); the same results. */
+#define MJH_MULTI_QUEUE 2
 /* mjh_multi_create with per-group choices (`tunes`: ngroups entries, or
  * NULL for the library's) and flags. */
 int mjh_multi_create_tuned(int ngroups, const mjhModel* models, const mjhData* datas, const mjhGroupTune* tunes,

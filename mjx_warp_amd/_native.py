@@ -98,6 +98,7 @@ class GroupTuneStruct(ctypes.Structure):
 
 
 MJH_MULTI_BUCKETS = 1
+MJH_MULTI_QUEUE = 2
 
 
 def model_sizes(m) -> dict:

@@ -650,7 +650,15 @@ struct GroupTable {
   GroupDesc* dev = nullptr;
   int ngroups = 0;
   int blocks = 0;
+  int* queue = nullptr;  // kQueueSlots launch counters (work-queue plans), or null
 };
+constexpr int kQueueSlots = 8;
+// workgroups of a work-queue launch: 8 per CU of a 256-CU device; more
+// than fit at once only take an empty ticket and exit
+constexpr int kQueueGrid = 2048;
+__global__ void k_zero_queue(int* q) {
+  if (threadIdx.x < kQueueSlots) q[threadIdx.x] = 0;
+}
 
 }  // namespace
 
@@ -692,16 +700,27 @@ int build_table(const std::vector<GroupDesc>& host, int wpb, GroupTable* t) {
 
 void free_tables(mjhMulti* p) {
   for (Bucket& b : p->buckets)
-    for (GroupTable* t : {&b.t32, &b.t32_64, &b.t64})
+    for (GroupTable* t : {&b.t32, &b.t32_64, &b.t64}) {
       if (t->dev) (void)hipFree(t->dev);
+      if (t->queue) (void)hipFree(t->queue);
+    }
 }
 
+// One launch of stage S over a group table: a workgroup per block, or, for
+// a table with queue counters, the work-queue form taking counter `slot`.
 template <int TEAM, class S, int TI>
-int launch_groups(const GroupTable& t, int ldsw, hipStream_t s, const char* name) {
+int launch_groups(const GroupTable& t, int ldsw, hipStream_t s, const char* name, int slot = -1) {
   if (t.ngroups == 0 || t.blocks == 0) return MJH_OK;
   constexpr int wpb = 64 / TEAM;
   if (wpb > 1 && ldsw % 4 != 0) return fail(MJH_ERR_INVALID, std::string(name) + ": LDS stride not 16-byte aligned");
   const size_t bytes = (size_t)ldsw * wpb * sizeof(float);
+  if (t.queue && slot >= 0 && slot < kQueueSlots) {
+    TRY(allow_lds((const void*)k_stage_queue<TEAM, S, TI>, bytes, name));
+    const int grid = t.blocks < kQueueGrid ? t.blocks : kQueueGrid;
+    hipLaunchKernelGGL((k_stage_queue<TEAM, S, TI>), dim3(grid), dim3(64), bytes, s, t.dev, t.ngroups, ldsw,
+                       t.queue + slot, t.blocks);
+    return check_launch(name);
+  }
   TRY(allow_lds((const void*)k_stage_groups<TEAM, S, TI>, bytes, name));
   hipLaunchKernelGGL((k_stage_groups<TEAM, S, TI>), dim3(t.blocks), dim3(64), bytes, s, t.dev, t.ngroups, ldsw);
   return check_launch(name);
@@ -895,28 +914,33 @@ int multi_launches(const Bucket& p, hipStream_t s, bool use_side) {
     hipLaunchKernelGGL(k_reset_groups, dim3(t->ngroups), dim3(64), 0, s, t->dev);
     TRY(check_launch("multi: reset"));
   }
+  for (const GroupTable* t : {&p.t32, &p.t32_64, &p.t64})
+    if (t->ngroups && t->queue) {
+      hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, s, t->queue);
+      TRY(check_launch("multi: queue counters"));
+    }
   if (p.t32.ngroups) {
-    TRY((launch_groups<kRegTeam, StFusedForward, 0>(p.t32, p.lds_fwd, s, "multi: forward")));
+    TRY((launch_groups<kRegTeam, StFusedForward, 0>(p.t32, p.lds_fwd, s, "multi: forward", 0)));
     TRY(beside(
         s,
         [&](hipStream_t q) {
-          return launch_groups<kRegTeam, StBigWorlds<true>, 1>(p.t32, p.lds_big, q, "multi: big worlds");
+          return launch_groups<kRegTeam, StBigWorlds<true>, 1>(p.t32, p.lds_big, q, "multi: big worlds", 1);
         },
         [&](hipStream_t q) {
-          return launch_groups<kRegTeam, StFusedSolveIntegrate, 0>(p.t32, p.lds_solve, q, "multi: solve");
+          return launch_groups<kRegTeam, StFusedSolveIntegrate, 0>(p.t32, p.lds_solve, q, "multi: solve", 2);
         },
         use_side));
-    TRY((launch_groups<kTeam, StTail<true>, 0>(p.t32_64, p.lds_tail32, s, "multi: tail")));
+    TRY((launch_groups<kTeam, StTail<true>, 0>(p.t32_64, p.lds_tail32, s, "multi: tail", 0)));
   }
   if (p.t64.ngroups) {
     if (split64()) {
-      TRY((launch_groups<kTeam, StForwardFactor, 0>(p.t64, p.lds_fwd64, s, "multi: forward")));
-      TRY((launch_groups<kTeam, StSolveStep, 0>(p.t64, p.lds_solve64, s, "multi: solve")));
-      TRY((launch_groups<kTeam, StIntegrateSolved, 0>(p.t64, p.lds_tail64, s, "multi: integrate")));
+      TRY((launch_groups<kTeam, StForwardFactor, 0>(p.t64, p.lds_fwd64, s, "multi: forward", 0)));
+      TRY((launch_groups<kTeam, StSolveStep, 0>(p.t64, p.lds_solve64, s, "multi: solve", 1)));
+      TRY((launch_groups<kTeam, StIntegrateSolved, 0>(p.t64, p.lds_tail64, s, "multi: integrate", 2)));
     } else {
-      TRY((launch_groups<kTeam, StFusedStep, 0>(p.t64, p.lds_step, s, "multi: step")));
+      TRY((launch_groups<kTeam, StFusedStep, 0>(p.t64, p.lds_step, s, "multi: step", 3)));
     }
-    TRY((launch_groups<kTeam, StTail<true>, 0>(p.t64, p.lds_tail64, s, "multi: tail")));
+    TRY((launch_groups<kTeam, StTail<true>, 0>(p.t64, p.lds_tail64, s, "multi: tail", 4)));
   }
   return MJH_OK;
 }
@@ -1084,7 +1108,7 @@ int mjh_multi_create_tuned(int ngroups, const mjhModel* models, const mjhData* d
   if (!out) return fail(MJH_ERR_INVALID, "multi: null output");
   *out = nullptr;
   if (ngroups <= 0 || !models || !datas) return fail(MJH_ERR_INVALID, "multi: no groups");
-  if (flags & ~MJH_MULTI_BUCKETS) return fail(MJH_ERR_INVALID, "multi: unknown flags");
+  if (flags & ~(MJH_MULTI_BUCKETS | MJH_MULTI_QUEUE)) return fail(MJH_ERR_INVALID, "multi: unknown flags");
   std::vector<Bucket> buckets;
   std::vector<std::vector<GroupDesc>> g32, g64;
   std::map<std::pair<int, int>, int> index;
@@ -1119,6 +1143,10 @@ int mjh_multi_create_tuned(int ngroups, const mjhModel* models, const mjhData* d
     rc = build_table(g32[k], 2, &b.t32);
     if (rc == MJH_OK) rc = build_table(g32[k], 1, &b.t32_64);
     if (rc == MJH_OK) rc = build_table(g64[k], 1, &b.t64);
+    if (flags & MJH_MULTI_QUEUE)
+      for (GroupTable* t : {&b.t32, &b.t32_64, &b.t64})
+        if (rc == MJH_OK && t->ngroups)
+          rc = check_hip(hipMalloc(&t->queue, kQueueSlots * sizeof(int)), "multi: queue counters");
   }
   if (rc != MJH_OK) {
     free_tables(p);

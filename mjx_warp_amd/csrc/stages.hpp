@@ -310,6 +310,27 @@ __global__ void __launch_bounds__(64) k_stage_groups(const GroupDesc* __restrict
   S()(G.m, G.d, smem + t.slot * lds_per_world, t, G.tune[TI]);
 }
 
+// The work-queue form of k_stage_groups: a grid of at most a few
+// workgroups per SIMD, each taking the next block index from a counter in
+// device memory (queue, zeroed before the launch) until all nblocks are
+// taken.  Blocks are taken in index order, largest models first; every
+// workgroup exits once the counter passes nblocks.  One wavefront per
+// workgroup, so lane 0's ticket is broadcast by a shuffle.
+template <int TEAM, class S, int TI>
+__global__ void __launch_bounds__(64) k_stage_queue(const GroupDesc* __restrict__ g, int ngroups, int lds_per_world,
+                                                    int* queue, int nblocks) {
+  extern __shared__ float smem[];
+  for (;;) {
+    int ticket = 0;
+    if (threadIdx.x == 0) ticket = atomicAdd(queue, 1);
+    const int b = __shfl(ticket, 0);
+    if (b >= nblocks) return;
+    const GroupDesc& G = group_of(g, ngroups, b);
+    Team<TEAM> t(G.d.nworld, b - G.first_block);
+    S()(G.m, G.d, smem + t.slot * lds_per_world, t, G.tune[TI]);
+    __syncthreads();
+  }
+}
 
 using StBigWorldsSolve = StBigWorlds<false>;
 using StBigWorldsStep = StBigWorlds<true>;
@@ -358,9 +379,12 @@ using StTailStep = StTail<true>;
   MJH_KERNELS_BIG_STEP_G(X, G) MJH_KERNELS_TAIL_STEP(X, G) MJH_KERNELS_FUSED64(X, G) MJH_KERNELS_SPLIT64(X, G)
 
 #define MJH_INSTANTIATE_STAGE(TEAM, S) template __global__ void k_stage<TEAM, S>(mjhModel, mjhData, int, Tune);
-#define MJH_INSTANTIATE_GROUPS(TEAM, S, TI) \
-  template __global__ void k_stage_groups<TEAM, S, TI>(const GroupDesc* __restrict__, int, int);
+#define MJH_INSTANTIATE_GROUPS(TEAM, S, TI)                                                   \
+  template __global__ void k_stage_groups<TEAM, S, TI>(const GroupDesc* __restrict__, int, int); \
+  template __global__ void k_stage_queue<TEAM, S, TI>(const GroupDesc* __restrict__, int, int, int*, int);
 #define MJH_EXTERN_STAGE(TEAM, S) extern MJH_INSTANTIATE_STAGE(TEAM, S)
-#define MJH_EXTERN_GROUPS(TEAM, S, TI) extern MJH_INSTANTIATE_GROUPS(TEAM, S, TI)
+#define MJH_EXTERN_GROUPS(TEAM, S, TI)                                                               \
+  extern template __global__ void k_stage_groups<TEAM, S, TI>(const GroupDesc* __restrict__, int, int); \
+  extern template __global__ void k_stage_queue<TEAM, S, TI>(const GroupDesc* __restrict__, int, int, int*, int);
 
 }  // namespace mjh

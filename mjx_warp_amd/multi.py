@@ -19,13 +19,15 @@ two worlds per wavefront with the register solver, nv <= 32 only; 64: one
 world per wavefront with the LDS solver), the LDS solver's row capacity and
 the Jacobian rows staged in LDS -- (team, rmax, rj), 0 / 0 / -1 for the
 library's choice; `buckets=True` launches groups of different LDS
-occupancy classes separately, each launch sized for its own groups.  Every
+occupancy classes separately, each launch sized for its own groups;
+`queue=True` makes each launch a work queue (a grid of a few workgroups per
+SIMD taking worlds, largest models first, from a device counter).  Every
 choice computes the same step.
 
 `autotune=True` picks them by timing on the groups' own state (restored
 after every trial): first each group alone over its candidate choices
 (`candidates`), then the whole batch grouped with and without buckets and
-per group; the fastest is kept, remembered per group signature (each
+work queues, and per group; the fastest is kept, remembered per group signature (each
 group's sizes and world count) for the rest of the process.
 
 The plan holds the groups' device pointers: write Data fields in place
@@ -66,7 +68,8 @@ def _signature(groups) -> tuple:
 class MultiStep:
   """One fused step of every world of every (Model, Data) group per call."""
 
-  def __init__(self, groups, autotune: bool = False, mode: str = "grouped", tunes=None, buckets: bool = False):
+  def __init__(self, groups, autotune: bool = False, mode: str = "grouped", tunes=None, buckets: bool = False,
+               queue: bool = False):
     if not groups:
       raise ValueError("MultiStep needs at least one (Model, Data) group")
     if mode not in ("grouped", "per_group", "static"):
@@ -77,6 +80,7 @@ class MultiStep:
     self.mode = mode
     self.tunes = list(tunes) if tunes is not None else None
     self.buckets = bool(buckets)
+    self.queue = bool(queue)
     self._plan = None
     self._structs = None
     self._build()
@@ -84,7 +88,7 @@ class MultiStep:
       key = _signature(self.groups)
       if key not in _TUNED:
         _TUNED[key] = self._tune()
-      self.mode, self.tunes, self.buckets = _TUNED[key]
+      self.mode, self.tunes, self.buckets, self.queue = _TUNED[key]
       self._build()
 
   def _build(self):
@@ -103,8 +107,8 @@ class MultiStep:
     if self.tunes is not None:
       tunes = (_native.GroupTuneStruct * n)(*[_native.GroupTuneStruct(*map(int, t)) for t in self.tunes])
     plan = ctypes.c_void_p()
-    rc = lib.mjh_multi_create_tuned(n, M, D, tunes, _native.MJH_MULTI_BUCKETS if self.buckets else 0,
-                                    ctypes.byref(plan))
+    flags = (_native.MJH_MULTI_BUCKETS if self.buckets else 0) | (_native.MJH_MULTI_QUEUE if self.queue else 0)
+    rc = lib.mjh_multi_create_tuned(n, M, D, tunes, flags, ctypes.byref(plan))
     if rc != _native.MJH_OK:
       _native._raise(rc, "mjh_multi_create_tuned")
     self._plan = plan
@@ -144,7 +148,7 @@ class MultiStep:
     return time.perf_counter() - t0
 
   def _tune(self, steps: int = 5) -> tuple:
-    """(mode, tunes, buckets) of the fastest trial."""
+    """(mode, tunes, buckets, queue) of the fastest trial."""
     import torch
 
     saved = [({f: getattr(d, f).clone() for f in _STATE}, d.time) for _, d in self.groups]
@@ -173,16 +177,18 @@ class MultiStep:
         if dt < (best_t if cand == (0, 0, -1) or best_t == float("inf") else 0.97 * best_t):
           best, best_t = cand, dt
       tunes.append(best)
-    # the whole batch: grouped with and without buckets, per group
-    trials = [("grouped", tunes, False), ("grouped", tunes, True), ("per_group", None, False)]
+    # the whole batch: grouped with and without buckets, each with and
+    # without work-queue launches, and per group
+    trials = [("grouped", tunes, bk, qu) for bk in (False, True) for qu in (False, True)]
+    trials.append(("per_group", None, False, False))
     best, best_t = trials[0], float("inf")
-    for mode, tn, bk in trials:
-      self.mode, self.tunes, self.buckets = mode, tn, bk
+    for mode, tn, bk, qu in trials:
+      self.mode, self.tunes, self.buckets, self.queue = mode, tn, bk, qu
       self._build()
       dt = self._timed(steps)
       restore(everything)
       if dt < best_t:
-        best, best_t = (mode, tn, bk), dt
+        best, best_t = (mode, tn, bk, qu), dt
     torch.cuda.synchronize()
     return best
 

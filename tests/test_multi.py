@@ -203,6 +203,36 @@ def test_buckets_with_library_choices_change_nothing():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("buckets", [False, True])
+def test_work_queue_launches_change_nothing(buckets):
+  """Work-queue launches (a grid of a few workgroups per SIMD taking worlds
+  from a device counter) compute the same step bit for bit as one
+  workgroup per world, 32- and 64-lane groups, with and without buckets;
+  also when the plan's step is captured in a graph and replayed."""
+  _, make = _groups(seed=7)
+  ga, gb = make(), make()
+  a, b = mjw.MultiStep(ga, buckets=buckets), mjw.MultiStep(gb, buckets=buckets, queue=True)
+  for _ in range(3):
+    a.step()
+    b.step()
+  torch.cuda.synchronize()
+  g = torch.cuda.CUDAGraph()
+  side = torch.cuda.Stream()
+  side.wait_stream(torch.cuda.current_stream())
+  with torch.cuda.graph(g, stream=side):
+    b.step()
+  torch.cuda.current_stream().wait_stream(side)
+  for _ in range(2):
+    a.step()
+    g.replay()
+    torch.cuda.synchronize()
+    for i, ((_, da), (_, db)) in enumerate(zip(ga, gb)):
+      for f in _FIELDS:
+        x, y = helpers.data_numpy(da, f), helpers.data_numpy(db, f)
+        assert np.array_equal(x, y), (i, f, np.abs(x - y).max())
+
+
+@pytest.mark.gpu
 def test_invalid_choices_are_refused():
   _, make = _groups(seed=6)
   groups = make()
