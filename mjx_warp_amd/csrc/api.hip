@@ -653,10 +653,10 @@ struct GroupTable {
   int* queue = nullptr;  // kQueueSlots launch counters (work-queue plans), or null
 };
 constexpr int kQueueSlots = 8;
-// workgroups of a work-queue launch: 32 per CU of a 256-CU device, about
-// what the dispatcher keeps resident for the small worlds' launches; more
-// than fit at once only take an empty ticket and exit
-constexpr int kQueueGrid = 8192;
+// workgroups of a work-queue launch: 8 per CU of a 256-CU device; more
+// than fit at once only take an empty ticket and exit (a 32-per-CU grid
+// measured slower: more workgroups contending for the one counter)
+constexpr int kQueueGrid = 2048;
 __global__ void k_zero_queue(int* q) {
   if (threadIdx.x < kQueueSlots) q[threadIdx.x] = 0;
 }
