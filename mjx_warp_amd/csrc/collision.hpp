@@ -327,21 +327,47 @@ MJH_DEV void sap_broadphase(const mjhModel& m, const mjhData& d, float* lds, con
       range[i] = limit - i;
     }
   sync();
-  // per-world inclusive scan of the ranges (the reference scans all worlds)
-  if (t.valid && t.lane == 0) {
-    int* cs = d.sap_cumulative_sum + (size_t)w * ng;
-    int s = 0;
-    for (int i = 0; i < ng; i++) {
-      s += range[i];
-      cs[i] = s;
+  // per-world inclusive scan of the ranges (the reference scans all worlds),
+  // in chunks of TEAM; kept in LDS over the sorted keys (no longer needed)
+  int* cs = reinterpret_cast<int*>(key);
+  {
+    int run = 0;
+    for (int i0 = 0; i0 < ng; i0 += TEAM) {
+      const int i = i0 + t.lane;
+      const int r = (t.valid && i < ng) ? range[i] : 0;
+      int tot;
+      const int ex = team_exscan_int<TEAM>(r, t.lane, &tot);
+      if (t.valid && i < ng) {
+        cs[i] = run + ex + r;
+        d.sap_cumulative_sum[(size_t)w * ng + i] = run + ex + r;
+      }
+      run += tot;
     }
   }
+  sync();
+  // candidate pairs p in [0, P), P = cs[ng - 1], i-major: sorted geom i
+  // (cs[i - 1] <= p < cs[i]) and j = i + 1 + p - cs[i - 1]; lanes take
+  // pairs, so a plane whose range spans every geom costs no lane more than
+  // the others
+  const int P = (t.valid && ng > 0) ? cs[ng - 1] : 0;
+  auto pair_of = [&](int p, int* pi, int* pj) {
+    int lo = 0, hi = ng - 1;  // first i with cs[i] > p
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (cs[mid] > p) hi = mid;
+      else lo = mid + 1;
+    }
+    *pi = lo;
+    *pj = lo + 1 + p - (lo > 0 ? cs[lo - 1] : 0);
+  };
   // pass 1: count accepted pairs
   int count = 0;
   if (t.valid)
-    for (int i = t.lane; i < ng; i += TEAM)
-      for (int j = i + 1; j <= i + range[i]; j++)
-        if (geom_filter(m, val[i], val[j], fp) && sap_overlap(gs[i], gs[j])) count++;
+    for (int p = t.lane; p < P; p += TEAM) {
+      int i, j;
+      pair_of(p, &i, &j);
+      if (geom_filter(m, val[i], val[j], fp) && sap_overlap(gs[i], gs[j])) count++;
+    }
   const int total = team_sum_int<TEAM>(count);
   const int base = team_reserve<TEAM>(d.ncollision, t.valid ? total : 0);
   const int cap = d.nconmax;
@@ -350,20 +376,21 @@ MJH_DEV void sap_broadphase(const mjhModel& m, const mjhData& d, float* lds, con
     d.world_pair[2 * w + 1] = total <= 0 ? 0 : max(0, min(total, cap - base));
     if (base + total > cap) atomicAdd(d.overflow + 0, min(total, base + total - cap));
   }
-  // pass 2: write, ordered by (sorted i, j)
+  // pass 2: write, ordered by (sorted i, j) = by p
   int k = 0;
-  for (int i0 = 0; i0 < ng; i0 += TEAM) {
-    const int i = i0 + t.lane;
-    int c = 0;
-    if (t.valid && i < ng)
-      for (int j = i + 1; j <= i + range[i]; j++)
-        if (geom_filter(m, val[i], val[j], fp) && sap_overlap(gs[i], gs[j])) c++;
+  for (int p0 = 0; p0 < P; p0 += TEAM) {
+    const int p = p0 + t.lane;
+    int i = 0, j = 0;
+    bool acc = false;
+    if (t.valid && p < P) {
+      pair_of(p, &i, &j);
+      acc = geom_filter(m, val[i], val[j], fp) && sap_overlap(gs[i], gs[j]);
+    }
     int tot;
-    const int off = team_exscan_int<TEAM>(c, t.lane, &tot);
-    if (t.valid && i < ng && c > 0) {
+    const int off = team_exscan_bits<TEAM, 1>(acc ? 1 : 0, &tot);
+    if (acc) {
       PairWriter<TEAM> pw{m, d, base, cap, k + off};
-      for (int j = i + 1; j <= i + range[i]; j++)
-        if (geom_filter(m, val[i], val[j], fp) && sap_overlap(gs[i], gs[j])) pw.put(val[i], val[j], w);
+      pw.put(val[i], val[j], w);
     }
     k += tot;
   }
