@@ -273,11 +273,7 @@ int lds_tail(const mjhModel& m, bool wide) {
   const int a = lds_linalg(m), h = wide ? m.nv + chol_blocked_floats(m.nv) : m.nv * m.nv + m.nv;
   return a > h ? a : h;
 }
-int lds_sap(const mjhModel& m) {
-  int np = 1;
-  while (np < m.ngeom) np <<= 1;
-  return 2 * np + 4 + 4 * m.ngeom;
-}
+int lds_sap(const mjhModel& m) { return lds_sap_words(m.ngeom); }
 
 size_t lds_cap();
 bool reg_path(const mjhModel& m);

@@ -11,6 +11,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "linalg_reg.hpp"
 
 namespace mjh {
 
@@ -232,42 +233,99 @@ MJH_DEV void nxn_broadphase(const mjhModel& m, const mjhData& d, float* lds, con
   }
 }
 
-// Bitonic sort of n (key, value) pairs in LDS, ascending by (key, value);
-// the padded length np is a power of two and padding keys are +inf.
+// LDS radix sort of n (key, value) pairs by key, stable (the reference
+// sorts with wp.utils.radix_sort_pairs): four passes of 8-bit digits,
+// least significant first, each a 256-bucket histogram (LDS atomics), a
+// team scan of the buckets and a scatter of the keys in chunks of TEAM in
+// input order, a key's place among equal digits of its chunk taken from
+// eight ballots.  Keys are floats mapped to order-preserving unsigned
+// integers (-0 as +0); key2 / val2 are scratch of n, hist of 256.  The
+// result is in key / val (an even number of passes).
+MJH_DEV unsigned float_key(float f) {
+  unsigned u = __float_as_uint(f == 0.0f ? 0.0f : f);
+  return u ^ ((u >> 31) ? 0xffffffffu : 0x80000000u);
+}
+MJH_DEV float key_float(unsigned u) { return __uint_as_float(u ^ ((u >> 31) ? 0x80000000u : 0xffffffffu)); }
+
 template <int TEAM>
-MJH_DEV void bitonic_sort_lds(float* key, int* val, int np, const Team<TEAM>& t) {
-  for (int k = 2; k <= np; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = t.lane; i < np; i += TEAM) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const float ki = key[i], kj = key[ixj];
-          const int vi = val[i], vj = val[ixj];
-          const bool gt = (ki > kj) || (ki == kj && vi > vj);
-          const bool up = (i & k) == 0;
-          if (gt == up) {
-            key[i] = kj;
-            key[ixj] = ki;
-            val[i] = vj;
-            val[ixj] = vi;
-          }
-        }
+MJH_DEV void radix_sort_lds(unsigned* key, int* val, unsigned* key2, int* val2, int* hist, int n,
+                            const Team<TEAM>& t) {
+  const unsigned long long team = team_lanes<TEAM>();
+  const unsigned long long below = __lanemask_lt() & team;
+  for (int shift = 0; shift < 32; shift += 8) {
+    for (int b = t.lane; b < 256; b += TEAM) hist[b] = 0;
+    sync();
+    if (t.valid)
+      for (int i = t.lane; i < n; i += TEAM) atomicAdd(&hist[(key[i] >> shift) & 255u], 1);
+    sync();
+    {
+      constexpr int per = 256 / TEAM;
+      int local[per];
+      int sum = 0;
+#pragma unroll
+      for (int q = 0; q < per; q++) {
+        local[q] = hist[t.lane * per + q];
+        sum += local[q];
       }
-      sync();
+      int tot;
+      int run = team_exscan_int<TEAM>(sum, t.lane, &tot);
+#pragma unroll
+      for (int q = 0; q < per; q++) {
+        hist[t.lane * per + q] = run;
+        run += local[q];
+      }
     }
+    sync();
+    for (int i0 = 0; i0 < n; i0 += TEAM) {
+      const int i = i0 + t.lane;
+      const bool in = t.valid && i < n;
+      const unsigned k = in ? key[i] : 0u;
+      const int v = in ? val[i] : 0;
+      const unsigned dg = (k >> shift) & 255u;
+      unsigned long long peers = __ballot(in) & team;
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const unsigned long long m = __ballot((dg >> b) & 1u);
+        peers &= ((dg >> b) & 1u) ? m : ~m;
+      }
+      int dst = 0;
+      if (in) dst = hist[dg] + __popcll(peers & below);
+      wave_fence();
+      // the highest lane of each digit's peers moves the bucket on
+      if (in && (peers & ~(below | (1ull << (threadIdx.x & 63)))) == 0) hist[dg] += __popcll(peers);
+      if (in) {
+        key2[dst] = k;
+        val2[dst] = v;
+      }
+      wave_fence();
+    }
+    sync();
+    unsigned* tk = key;
+    key = key2;
+    key2 = tk;
+    int* tv = val;
+    val = val2;
+    val2 = tv;
   }
 }
 
-// collision_driver.py:350-455 sap_broadphase.  lds: 2*np words for the sort
-// plus 4*ngeom floats for the sorted bounding volumes; np = next pow2(ngeom).
+// LDS words of sap_broadphase: the sort's keys, values, their scratch
+// copies and 256 buckets, then (16-byte aligned) a float4 per geom for the
+// sorted bounding volumes.
+__host__ __device__ inline int sap_sort_words(int ng) { return (4 * ng + 256 + 3) & ~3; }
+__host__ __device__ inline int lds_sap_words(int ng) { return sap_sort_words(ng) + 4 * ng; }
+
+// collision_driver.py:350-455 sap_broadphase.  lds: lds_sap_words(ngeom).
 template <int TEAM>
 MJH_DEV void sap_broadphase(const mjhModel& m, const mjhData& d, float* lds, const Team<TEAM>& t) {
   const int w = t.world, ng = m.ngeom;
-  int np = 1;
-  while (np < ng) np <<= 1;
   float* key = lds;
-  int* val = reinterpret_cast<int*>(lds + np);
-  float4* gs = reinterpret_cast<float4*>(lds + 2 * np + ((2 * np) & 3 ? 4 - ((2 * np) & 3) : 0));
+  int* val = reinterpret_cast<int*>(lds + ng);
+  unsigned* ukey = reinterpret_cast<unsigned*>(key);
+  unsigned* key2 = reinterpret_cast<unsigned*>(lds + 2 * ng);
+  int* val2 = reinterpret_cast<int*>(lds + 3 * ng);
+  int* hist = reinterpret_cast<int*>(lds + 4 * ng);
+  float4* gs = reinterpret_cast<float4*>(lds + sap_sort_words(ng));
   const bool fp = !(m.opt.disableflags & mjhDSBL_FILTERPARENT);
   const float* gxpos = WPTR(d.geom_xpos, w, ng * 3);
   const float* gxmat = WPTR(d.geom_xmat, w, ng * 9);
@@ -276,23 +334,22 @@ MJH_DEV void sap_broadphase(const mjhModel& m, const mjhData& d, float* lds, con
   int* sidx = WPTR(d.sap_sort_index, w, ng);
   // fixed sweep direction normalize(0.5935, 0.7790, 0.1235)
   const V3 dir = normalize(V3{0.5935f, 0.7790f, 0.1235f});
-  for (int i = t.lane; i < np; i += TEAM) {
-    if (i < ng && t.valid) {
+  if (t.valid)
+    for (int i = t.lane; i < ng; i += TEAM) {
       const V3 c = ld3(gxpos + 3 * i);
       float r = m.geom_rbound[i];
       if (r == 0.0f) r = 1000000000.0f;
       const float rad = r + m.geom_margin[i];
       const float center = dot(dir, c);
-      key[i] = center - rad;
+      ukey[i] = float_key(center - rad);
       val[i] = i;
       upper[i] = center + rad;
-    } else {
-      key[i] = __int_as_float(0x7f800000);
-      val[i] = 0x7fffffff;
     }
-  }
   sync();
-  bitonic_sort_lds(key, val, np, t);
+  radix_sort_lds(ukey, val, key2, val2, hist, ng, t);
+  if (t.valid)
+    for (int i = t.lane; i < ng; i += TEAM) key[i] = key_float(ukey[i]);
+  sync();
   if (t.valid)
     for (int i = t.lane; i < ng; i += TEAM) {
       lower[i] = key[i];
