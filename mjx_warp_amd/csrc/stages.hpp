@@ -315,7 +315,10 @@ __global__ void __launch_bounds__(64) k_stage_groups(const GroupDesc* __restrict
 // device memory (queue, zeroed before the launch) until all nblocks are
 // taken.  Blocks are taken in index order, largest models first; every
 // workgroup exits once the counter passes nblocks.  One wavefront per
-// workgroup, so lane 0's ticket is broadcast by a shuffle.
+// workgroup, all lanes active at the read: lane 0's ticket is taken with
+// readfirstlane, which puts it in a scalar register, so the group lookup
+// and the model description are scalar loads as with blockIdx.x (a
+// shuffle leaves it in a vector register and every model pointer with it).
 template <int TEAM, class S, int TI>
 __global__ void __launch_bounds__(64) k_stage_queue(const GroupDesc* __restrict__ g, int ngroups, int lds_per_world,
                                                     int* queue, int nblocks) {
@@ -323,7 +326,7 @@ __global__ void __launch_bounds__(64) k_stage_queue(const GroupDesc* __restrict_
   for (;;) {
     int ticket = 0;
     if (threadIdx.x == 0) ticket = atomicAdd(queue, 1);
-    const int b = __shfl(ticket, 0);
+    const int b = __builtin_amdgcn_readfirstlane(ticket);
     if (b >= nblocks) return;
     const GroupDesc& G = group_of(g, ngroups, b);
     Team<TEAM> t(G.d.nworld, b - G.first_block);
