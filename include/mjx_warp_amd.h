@@ -500,9 +500,10 @@ typedef struct mjhGroupTune {
  * it every launch takes the largest group's LDS. */
 #define MJH_MULTI_BUCKETS 1
 /* Work-queue launches: each stage launch of the plan is a grid of a few
- * workgroups per SIMD that take worlds from a counter in device memory
- * (largest models first) until none are left, instead of one workgroup per
- * world (or pair of worlds); the same results. */
+ * workgroups per SIMD that take worlds from eight counters in device
+ * memory (each counter every eighth world, largest models first) until none
+ * are left, instead of one workgroup per world (or pair of worlds); the same
+ * results. */
 #define MJH_MULTI_QUEUE 2
 /* mjh_multi_create with per-group choices (`tunes`: ngroups entries, or
  * NULL for the library's) and flags. */

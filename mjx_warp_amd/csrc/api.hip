@@ -646,15 +646,19 @@ struct GroupTable {
   GroupDesc* dev = nullptr;
   int ngroups = 0;
   int blocks = 0;
-  int* queue = nullptr;  // kQueueSlots launch counters (work-queue plans), or null
+  int* queue = nullptr;  // kQueueSlots launches x kQueueLanes counters (work-queue plans), or null
 };
 constexpr int kQueueSlots = 8;
-// workgroups of a work-queue launch: 8 per CU of a 256-CU device; more
-// than fit at once only take an empty ticket and exit (a 32-per-CU grid
-// measured slower: more workgroups contending for the one counter)
+constexpr int kQueueInts = kQueueSlots * kQueueLanes * kQueueStride;
+// workgroups of a work-queue launch: 8 per CU of a 256-CU device (a
+// multiple of kQueueLanes); more than fit at once only take an empty
+// ticket and exit (a 32-per-CU grid measured slower: more workgroups
+// contending for the counters)
 constexpr int kQueueGrid = 2048;
+static_assert(kQueueGrid % kQueueLanes == 0, "queue grid covers every counter");
+static_assert(kQueueSlots * kQueueLanes == 64, "k_zero_queue clears one counter per lane");
 __global__ void k_zero_queue(int* q) {
-  if (threadIdx.x < kQueueSlots) q[threadIdx.x] = 0;
+  q[threadIdx.x * kQueueStride] = 0;
 }
 
 }  // namespace
@@ -713,9 +717,11 @@ int launch_groups(const GroupTable& t, int ldsw, hipStream_t s, const char* name
   const size_t bytes = (size_t)ldsw * wpb * sizeof(float);
   if (t.queue && slot >= 0 && slot < kQueueSlots) {
     TRY(allow_lds((const void*)k_stage_queue<TEAM, S, TI>, bytes, name));
-    const int grid = t.blocks < kQueueGrid ? t.blocks : kQueueGrid;
+    // a multiple of kQueueLanes, so every counter has workgroups drawing on it
+    const int want = t.blocks < kQueueGrid ? t.blocks : kQueueGrid;
+    const int grid = (want + kQueueLanes - 1) / kQueueLanes * kQueueLanes;
     hipLaunchKernelGGL((k_stage_queue<TEAM, S, TI>), dim3(grid), dim3(64), bytes, s, t.dev, t.ngroups, ldsw,
-                       t.queue + slot, t.blocks);
+                       t.queue + slot * kQueueLanes * kQueueStride, t.blocks);
     return check_launch(name);
   }
   TRY(allow_lds((const void*)k_stage_groups<TEAM, S, TI>, bytes, name));
@@ -1143,7 +1149,7 @@ int mjh_multi_create_tuned(int ngroups, const mjhModel* models, const mjhData* d
     if (flags & MJH_MULTI_QUEUE)
       for (GroupTable* t : {&b.t32, &b.t32_64, &b.t64})
         if (rc == MJH_OK && t->ngroups)
-          rc = check_hip(hipMalloc(&t->queue, kQueueSlots * sizeof(int)), "multi: queue counters");
+          rc = check_hip(hipMalloc(&t->queue, kQueueInts * sizeof(int)), "multi: queue counters");
   }
   if (rc != MJH_OK) {
     free_tables(p);

@@ -311,22 +311,31 @@ __global__ void __launch_bounds__(64) k_stage_groups(const GroupDesc* __restrict
 }
 
 // The work-queue form of k_stage_groups: a grid of at most a few
-// workgroups per SIMD, each taking the next block index from a counter in
-// device memory (queue, zeroed before the launch) until all nblocks are
-// taken.  Blocks are taken in index order, largest models first; every
-// workgroup exits once the counter passes nblocks.  One wavefront per
-// workgroup, all lanes active at the read: lane 0's ticket is taken with
-// readfirstlane, which puts it in a scalar register, so the group lookup
-// and the model description are scalar loads as with blockIdx.x (a
-// shuffle leaves it in a vector register and every model pointer with it).
+// workgroups per SIMD (a multiple of kQueueLanes), each taking the next
+// block index from one of kQueueLanes counters in device memory (queue,
+// kQueueStride ints apart, zeroed before the launch) until all nblocks are
+// taken.  Workgroup w draws from counter c = w % kQueueLanes, whose tickets
+// i name blocks c + kQueueLanes * i: every block once, each counter's in
+// index order, largest models first.  Atomics on one address serialise
+// (about 16 ns each here), so the tickets are spread over counters on
+// separate 256-byte lines.  Every workgroup exits once its counter passes
+// nblocks.  One wavefront per workgroup, all lanes active at the read:
+// lane 0's ticket is taken with readfirstlane, which puts it in a scalar
+// register, so the group lookup and the model description are scalar
+// loads as with blockIdx.x (a shuffle leaves it in a vector register and
+// every model pointer with it).
+constexpr int kQueueLanes = 8;
+constexpr int kQueueStride = 64;
 template <int TEAM, class S, int TI>
 __global__ void __launch_bounds__(64) k_stage_queue(const GroupDesc* __restrict__ g, int ngroups, int lds_per_world,
                                                     int* queue, int nblocks) {
   extern __shared__ float smem[];
+  const int c = blockIdx.x % kQueueLanes;
+  int* counter = queue + c * kQueueStride;
   for (;;) {
     int ticket = 0;
-    if (threadIdx.x == 0) ticket = atomicAdd(queue, 1);
-    const int b = __builtin_amdgcn_readfirstlane(ticket);
+    if (threadIdx.x == 0) ticket = atomicAdd(counter, 1);
+    const int b = c + kQueueLanes * __builtin_amdgcn_readfirstlane(ticket);
     if (b >= nblocks) return;
     const GroupDesc& G = group_of(g, ngroups, b);
     Team<TEAM> t(G.d.nworld, b - G.first_block);
