@@ -21,7 +21,7 @@ the Jacobian rows staged in LDS -- (team, rmax, rj), 0 / 0 / -1 for the
 library's choice; `buckets=True` launches groups of different LDS
 occupancy classes separately, each launch sized for its own groups;
 `queue=True` makes each launch a work queue (a grid of a few workgroups per
-SIMD taking worlds, largest models first, from a device counter).  Every
+SIMD taking worlds, largest models first, from eight device counters).  Every
 choice computes the same step.
 
 `autotune=True` picks them by timing on the groups' own state (restored
